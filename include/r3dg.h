@@ -1,0 +1,181 @@
+/*
+ * r3dg.h -- C ABI of the MI355X (gfx950) Census-SGM stereo matcher.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers, sizes and POD
+ * descriptors, never throws, and returns an r3dg_status.  The message of the
+ * last failure of a context is read with r3dg_last_error(); failures that
+ * happen before a context exists go to r3dg_global_error().
+ *
+ * Images are strided 2-D views (r3dg_image).  `memory` says whether `data`
+ * is host memory (the library copies through a pinned staging buffer) or
+ * device memory of the context's GPU (used in place, no copy).  Outputs follow
+ * the same rule.  All calls of one context run on that context's HIP stream
+ * and return after the results are complete, like the reference API they
+ * replace; several host threads may share a context (calls are serialised by
+ * a per-context mutex).
+ *
+ * Semantics follow the reference matcher (rhiestan/Regard3D_MVS,
+ * src/r3d/stereo/stereo_matcher.hpp); each declaration cites the reference
+ * interface it replaces.
+ */
+#ifndef R3DG_H_
+#define R3DG_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define R3DG_ABI_VERSION 1
+
+typedef enum r3dg_status {
+  R3DG_OK = 0,
+  R3DG_ERROR_INVALID_ARGUMENT = 1, /* maps to r3d::Error (stereo_matcher.cpp:81-111, 128-148) */
+  R3DG_ERROR_OUT_OF_MEMORY = 2,    /* device or pinned allocation failed */
+  R3DG_ERROR_DEVICE = 3,           /* a HIP runtime call failed */
+  R3DG_ERROR_CANCELLED = 4,        /* maps to r3d::Cancelled (core/cancel.hpp:10-13) */
+  R3DG_ERROR_BUDGET = 5,           /* request larger than the context's HBM budget */
+  R3DG_ERROR_NOT_FOUND = 6,        /* unknown ticket or view id */
+  R3DG_ERROR_INTERNAL = 7
+} r3dg_status;
+
+typedef enum r3dg_memory { R3DG_HOST = 0, R3DG_DEVICE = 1 } r3dg_memory;
+
+/* A strided single-channel image; stride in elements (>= width).  A view with
+ * data == NULL or width == 0 or height == 0 is empty (masks: all valid). */
+typedef struct r3dg_image {
+  void* data;
+  int32_t width;
+  int32_t height;
+  int64_t stride;
+  int32_t memory; /* r3dg_memory */
+  int32_t reserved;
+} r3dg_image;
+
+typedef enum r3dg_subpixel {
+  R3DG_SUBPIXEL_NONE = 0,
+  R3DG_SUBPIXEL_PARABOLA = 1,
+  R3DG_SUBPIXEL_EQUIANGULAR = 2
+} r3dg_subpixel;
+
+/* Mirror of r3d::StereoOptions (stereo_matcher.hpp:30-57); the reference's
+ * `scalar_kernels` switch selects CPU kernels and has no counterpart here. */
+typedef struct r3dg_options {
+  int32_t p1;
+  int32_t p2;
+  int32_t p2_adapt;
+  int32_t invalid_cost;
+  int32_t subpixel; /* r3dg_subpixel */
+  float uniqueness;
+  int32_t lr_max_diff;
+  int32_t lr_full_match;
+  int32_t speckle_size;
+  float speckle_diff;
+  int32_t median;
+  int32_t levels;
+  int32_t range_radius;
+  int32_t range_margin;
+  int32_t range_fill;
+  int32_t tile_size;
+  int32_t tile_margin;
+  int32_t tile_blend;
+  uint64_t memory_budget;
+} r3dg_options;
+
+/* Per-call statistics (DisparityMap fields, stereo_matcher.hpp:61-68, plus timings). */
+typedef struct r3dg_match_info {
+  int32_t disparity_min;   /* DisparityMap::disparity_min */
+  int32_t num_disparities; /* DisparityMap::num_disparities (aligned to 16) */
+  int32_t levels;          /* DisparityMap::levels */
+  int32_t tiles_level0;    /* tiles of the finest level, one direction */
+  uint64_t cells;          /* DisparityMap::cells: aggregated cells, all levels and directions */
+  uint64_t tile_pixels;    /* sum of tile pixels over levels and directions */
+  double seconds;          /* wall time of the call */
+} r3dg_match_info;
+
+/* Optional integer outputs of r3dg_match at level 0 (views may be empty).
+ * `labels`: int16 WTA disparity (disparity_min + best label) of the tile that
+ * dominates the blend of each pixel, INT16_MIN where the blend is invalid.
+ * `lr_mask`: uint8, 1 where the left-right check invalidated a valid pixel. */
+typedef struct r3dg_match_debug {
+  r3dg_image labels;
+  r3dg_image lr_mask;
+} r3dg_match_debug;
+
+typedef struct r3dg_ctx r3dg_ctx;
+
+/* ---- library and context ------------------------------------------------ */
+
+int32_t r3dg_abi_version(void);
+
+/* Defaults of r3d::StereoOptions (stereo_matcher.hpp:30-57). */
+void r3dg_default_options(r3dg_options* options);
+
+/* Creates a context on HIP device `device`; `hbm_budget` bytes (0: 90% of the
+ * free memory) bound the device memory the context may hold.  No reference
+ * counterpart: the reference runs on the process-wide CPU thread pool
+ * (core/thread_pool.hpp:65-77) with the per-call MemoryGate
+ * (stereo_matcher.cpp:580-601). */
+int32_t r3dg_ctx_create(int32_t device, uint64_t hbm_budget, r3dg_ctx** out);
+int32_t r3dg_ctx_destroy(r3dg_ctx* ctx);
+
+/* Copies the message of the context's last failure into buf (NUL-terminated)
+ * and returns the status of that failure. */
+int32_t r3dg_last_error(const r3dg_ctx* ctx, char* buf, size_t size);
+int32_t r3dg_global_error(char* buf, size_t size);
+
+/* Cancellation flag polled between pyramid levels and tile stages; replaces
+ * CancellationToken polling in parallel_for (core/thread_pool.cpp:41-70). */
+int32_t r3dg_cancel(r3dg_ctx* ctx, int32_t requested);
+
+/* ---- matcher stages ------------------------------------------------------ */
+
+/* census_transform (stereo/census.hpp:20): 9x7 Census, 62 bits, first
+ * neighbour in bit 61, replicated borders.  img: uint16; out: uint64 image of
+ * the same size. */
+int32_t r3dg_census(r3dg_ctx* ctx, r3dg_image img, r3dg_image out);
+
+/* pyramid_down<uint16_t/uint8_t> (core/pyramid.hpp:20): [1 4 6 4 1]/16
+ * separable decimation, (sum + 128) >> 8; out is ceil(w/2) x ceil(h/2). */
+int32_t r3dg_pyramid_down_u16(r3dg_ctx* ctx, r3dg_image img, r3dg_image out);
+int32_t r3dg_pyramid_down_u8(r3dg_ctx* ctx, r3dg_image img, r3dg_image out);
+
+/* detail::sgm_cost_volume (stereo_matcher.hpp:98-101): the whole image as one
+ * tile; S of every pixel's labels in row-major pixel order.  lo/num are uint16
+ * images of the left size (both empty: full range).  out_cells must equal the
+ * sum of num (w * h * align16(num_disparities) for the full range). */
+int32_t r3dg_cost_volume(r3dg_ctx* ctx, r3dg_image left, r3dg_image right, int32_t disparity_min,
+                         int32_t num_disparities, const r3dg_options* options, r3dg_image lo, r3dg_image num,
+                         uint16_t* out, uint64_t out_cells, int32_t out_memory);
+
+/* match_stereo (stereo_matcher.hpp:82-85).  left/right: uint16; masks: uint8
+ * (255 valid, empty view: all valid); disparity/confidence: float images of
+ * the left size (confidence may be empty); debug may be NULL. */
+int32_t r3dg_match(r3dg_ctx* ctx, r3dg_image left, r3dg_image right, r3dg_image left_valid,
+                   r3dg_image right_valid, int32_t disparity_min, int32_t num_disparities,
+                   const r3dg_options* options, r3dg_image disparity, r3dg_image confidence,
+                   const r3dg_match_debug* debug, r3dg_match_info* info);
+
+/* remove_speckles (stereo_matcher.hpp:89), in place on a float image. */
+int32_t r3dg_remove_speckles(r3dg_ctx* ctx, r3dg_image disparity, int32_t min_size, float max_diff);
+
+/* median_filter_3x3 (stereo_matcher.hpp:91). */
+int32_t r3dg_median_filter_3x3(r3dg_ctx* ctx, r3dg_image disparity, r3dg_image out);
+
+/* detail::ranges_from_coarse (stereo_matcher.hpp:121-125); lo/num: uint16
+ * images of width x height. */
+int32_t r3dg_ranges_from_coarse(r3dg_ctx* ctx, r3dg_image coarse, int32_t width, int32_t height,
+                                r3dg_image fine_valid, r3dg_image partner_valid, int32_t partner_width,
+                                int32_t partner_sign, int32_t disparity_min, int32_t num_disparities,
+                                const r3dg_options* options, int32_t mirrored, r3dg_image lo, r3dg_image num);
+
+/* detail::automatic_levels (stereo_matcher.hpp:128). Host-only arithmetic. */
+int32_t r3dg_automatic_levels(int32_t width, int32_t height, int32_t num_disparities);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* R3DG_H_ */

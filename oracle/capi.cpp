@@ -1,0 +1,154 @@
+// extern "C" entry points of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+// Contiguous arrays (stride == width); the Python side is oracle/oracle.py.
+#include <cstring>
+#include <string>
+
+#include "oracle.hpp"
+
+namespace {
+thread_local std::string g_error;
+
+template <typename T>
+oracle::Img<T> copy_in(const T* p, int w, int h) {
+  oracle::Img<T> img;
+  if (p == nullptr || w <= 0 || h <= 0) return img;
+  img = oracle::Img<T>(w, h);
+  std::memcpy(img.px.data(), p, sizeof(T) * img.px.size());
+  return img;
+}
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    g_error = e.what();
+    return 1;
+  }
+}
+
+oracle::Ranges ranges_in(const std::uint16_t* lo, const std::uint16_t* num, int w, int h) {
+  return oracle::Ranges{copy_in(lo, w, h), copy_in(num, w, h)};
+}
+}  // namespace
+
+extern "C" {
+
+const char* oracle_last_error(void) { return g_error.c_str(); }
+
+int oracle_census(const std::uint16_t* img, int w, int h, std::uint64_t* out) {
+  return guarded([&] {
+    const auto c = oracle::census(copy_in(img, w, h));
+    std::memcpy(out, c.px.data(), sizeof(std::uint64_t) * c.px.size());
+  });
+}
+
+int oracle_pyramid_down_u16(const std::uint16_t* img, int w, int h, std::uint16_t* out) {
+  return guarded([&] {
+    const auto p = oracle::pyramid_down(copy_in(img, w, h));
+    std::memcpy(out, p.px.data(), sizeof(std::uint16_t) * p.px.size());
+  });
+}
+
+int oracle_pyramid_down_u8(const std::uint8_t* img, int w, int h, std::uint8_t* out) {
+  return guarded([&] {
+    const auto p = oracle::pyramid_down(copy_in(img, w, h));
+    std::memcpy(out, p.px.data(), p.px.size());
+  });
+}
+
+int oracle_reduce_mask(const std::uint8_t* img, int w, int h, std::uint8_t* out) {
+  return guarded([&] {
+    const auto p = oracle::reduce_mask(copy_in(img, w, h));
+    std::memcpy(out, p.px.data(), p.px.size());
+  });
+}
+
+int oracle_automatic_levels(int w, int h, int nd) { return oracle::automatic_levels(w, h, nd); }
+
+static int volume(bool textbook, const std::uint16_t* left, int wl, int h, const std::uint16_t* right, int wr,
+                  int dmin, int nd, const r3dg_options* o, const std::uint16_t* lo, const std::uint16_t* num,
+                  std::uint16_t* out, std::uint64_t cells) {
+  return guarded([&] {
+    const auto L = copy_in(left, wl, h);
+    const auto R = copy_in(right, wr, h);
+    const bool has = lo != nullptr && num != nullptr;
+    const oracle::Ranges r = has ? ranges_in(lo, num, wl, h) : oracle::Ranges{};
+    const auto S = textbook ? oracle::textbook_volume(L, R, dmin, nd, *o, has ? &r : nullptr)
+                            : oracle::cost_volume(L, R, dmin, nd, *o, has ? &r : nullptr);
+    if (S.size() != cells) throw oracle::Error("oracle: output size does not match the cell count");
+    std::memcpy(out, S.data(), sizeof(std::uint16_t) * S.size());
+  });
+}
+
+int oracle_cost_volume(const std::uint16_t* left, int wl, int h, const std::uint16_t* right, int wr, int dmin,
+                       int nd, const r3dg_options* o, const std::uint16_t* lo, const std::uint16_t* num,
+                       std::uint16_t* out, std::uint64_t cells) {
+  return volume(false, left, wl, h, right, wr, dmin, nd, o, lo, num, out, cells);
+}
+
+int oracle_textbook_volume(const std::uint16_t* left, int wl, int h, const std::uint16_t* right, int wr, int dmin,
+                           int nd, const r3dg_options* o, const std::uint16_t* lo, const std::uint16_t* num,
+                           std::uint16_t* out, std::uint64_t cells) {
+  return volume(true, left, wl, h, right, wr, dmin, nd, o, lo, num, out, cells);
+}
+
+int oracle_wta_row(const std::uint16_t* sum, int w, const std::int32_t* lo, const std::uint16_t* num,
+                   std::uint16_t* best, std::uint16_t* best_cost, std::uint16_t* second, std::uint16_t* right_cost,
+                   std::uint16_t* right_best, std::int64_t right_shift) {
+  return guarded([&] { oracle::wta_row(sum, w, lo, num, best, best_cost, second, right_cost, right_best, right_shift); });
+}
+
+int oracle_ranges_from_coarse(const float* coarse, int wc, int hc, int width, int height, const std::uint8_t* fine_valid,
+                              const std::uint8_t* partner_valid, int partner_width, int partner_sign, int dmin,
+                              int nd, const r3dg_options* o, int mirrored, std::uint16_t* lo, std::uint16_t* num) {
+  return guarded([&] {
+    const auto r = oracle::ranges_from_coarse(copy_in(coarse, wc, hc), width, height, copy_in(fine_valid, width, height),
+                                              copy_in(partner_valid, partner_width, height), partner_width,
+                                              partner_sign, dmin, nd, *o, mirrored != 0);
+    std::memcpy(lo, r.lo.px.data(), sizeof(std::uint16_t) * r.lo.px.size());
+    std::memcpy(num, r.num.px.data(), sizeof(std::uint16_t) * r.num.px.size());
+  });
+}
+
+int oracle_remove_speckles(float* d, int w, int h, int min_size, float max_diff) {
+  return guarded([&] {
+    auto img = copy_in(d, w, h);
+    oracle::remove_speckles(img, min_size, max_diff);
+    if (!img.px.empty()) std::memcpy(d, img.px.data(), sizeof(float) * img.px.size());
+  });
+}
+
+int oracle_median3x3(const float* d, int w, int h, float* out) {
+  return guarded([&] {
+    const auto m = oracle::median3x3(copy_in(d, w, h));
+    if (!m.px.empty()) std::memcpy(out, m.px.data(), sizeof(float) * m.px.size());
+  });
+}
+
+int oracle_match(const std::uint16_t* left, int wl, int h, const std::uint16_t* right, int wr,
+                 const std::uint8_t* left_valid, const std::uint8_t* right_valid, int dmin, int nd,
+                 const r3dg_options* o, int threads, float* disparity, float* confidence, std::int16_t* labels,
+                 std::uint8_t* lr_mask, r3dg_match_info* info) {
+  return guarded([&] {
+    const auto r = oracle::match_stereo(copy_in(left, wl, h), copy_in(right, wr, h), copy_in(left_valid, wl, h),
+                                        copy_in(right_valid, wr, h), dmin, nd, *o, threads);
+    const std::size_t n = std::size_t(wl) * std::size_t(h);
+    std::memcpy(disparity, r.disparity.px.data(), sizeof(float) * n);
+    if (confidence != nullptr) std::memcpy(confidence, r.confidence.px.data(), sizeof(float) * n);
+    if (labels != nullptr) std::memcpy(labels, r.labels.px.data(), sizeof(std::int16_t) * n);
+    if (lr_mask != nullptr) std::memcpy(lr_mask, r.lr_mask.px.data(), n);
+    if (info != nullptr) {
+      info->disparity_min = r.disparity_min;
+      info->num_disparities = r.num_disparities;
+      info->levels = r.levels;
+      info->tiles_level0 = r.tiles_level0;
+      info->cells = r.cells;
+      info->tile_pixels = r.tile_pixels;
+      info->seconds = 0.0;
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,116 @@
+// CPU oracle of the Census-SGM matcher -- TEST INFRASTRUCTURE ONLY.
+//
+// A restatement, in plain C++, of the reference matcher
+// (rhiestan/Regard3D_MVS, src/r3d/stereo/{census,sgm_kernels,stereo_matcher}.cpp
+// and src/r3d/core/pyramid.cpp).  It is the checker of the GPU library and the
+// CPU baseline of bench.py; only tests/, __graft_entry__.smoke() and bench.py's
+// CPU-baseline leg may load it.  The product never links it.
+//
+// Pinning (how each part is checked against something other than itself):
+//   census            KATs of tests/test_census.cpp:26-65, brute force in numpy
+//   pyramid_down      closed form (u8 pyramid == rounded float pyramid,
+//                     test_pyramid.cpp:143-176), flat images stay flat
+//   aggregate_tile    == textbook_volume (an independent 8-sweep restatement of
+//                     stereo_matcher.cpp:1280-1362), as test_stereo.cpp:87-151
+//   wta_row           brute force (first argmin, second outside best +- 1)
+//   ranges_from_coarse KATs of test_stereo.cpp:343-400
+//   remove_speckles   KATs of test_stereo.cpp:451-469 + connected components
+//   median_filter_3x3 KATs of test_stereo.cpp:451-469 + sorted-window median
+//   match_stereo      behaviour thresholds of test_stereo.cpp:191-449
+// Floating-point steps (sub-pixel fit, blending, LR rounding) are restated
+// operation by operation and compiled with -ffp-contract=off.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "r3dg.h"
+
+namespace oracle {
+
+struct Error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+template <typename T>
+struct Img {
+  int w = 0;
+  int h = 0;
+  std::vector<T> px;
+  Img() = default;
+  Img(int w_, int h_, T v = T{}) : w(w_), h(h_), px(static_cast<std::size_t>(w_) * static_cast<std::size_t>(h_), v) {}
+  T& operator()(int x, int y) { return px[static_cast<std::size_t>(y) * w + x]; }
+  const T& operator()(int x, int y) const { return px[static_cast<std::size_t>(y) * w + x]; }
+  T* row(int y) { return px.data() + static_cast<std::size_t>(y) * w; }
+  const T* row(int y) const { return px.data() + static_cast<std::size_t>(y) * w; }
+  [[nodiscard]] bool empty() const { return w == 0 || h == 0; }
+};
+using ImgU8 = Img<std::uint8_t>;
+using ImgU16 = Img<std::uint16_t>;
+using ImgF = Img<float>;
+using ImgU64 = Img<std::uint64_t>;
+
+// Per-pixel label ranges (LabelRanges, stereo_matcher.hpp:73-76).
+struct Ranges {
+  ImgU16 lo;
+  ImgU16 num;
+};
+
+constexpr float kInvalid = __builtin_huge_valf();
+
+int align16(int n);
+int floor_shift(int v, int k);
+int ceil_shift(int v, int k);
+
+// Census (census.cpp:20-35, 104-113, 121-164).
+ImgU64 census(const ImgU16& img);
+
+// pyramid_down (pyramid.cpp:34-85) and reduce_mask (stereo_matcher.cpp:892-900).
+ImgU16 pyramid_down(const ImgU16& img);
+ImgU8 pyramid_down(const ImgU8& img);
+ImgU8 reduce_mask(const ImgU8& mask);
+
+int automatic_levels(int width, int height, int num_disparities);
+
+// Aggregated volume of the whole image as one tile (sgm_cost_volume,
+// stereo_matcher.cpp:1269-1278) through the row-pass dataflow.
+std::vector<std::uint16_t> cost_volume(const ImgU16& left, const ImgU16& right, int dmin, int nd,
+                                       const r3dg_options& o, const Ranges* ranges);
+// Textbook volume (stereo_matcher.cpp:1280-1362): 8 separate path sweeps in int.
+std::vector<std::uint16_t> textbook_volume(const ImgU16& left, const ImgU16& right, int dmin, int nd,
+                                           const r3dg_options& o, const Ranges* ranges);
+
+// WTA of one row (sgm_kernels.cpp:405-438).
+void wta_row(const std::uint16_t* sum, int width, const std::int32_t* lo, const std::uint16_t* num,
+             std::uint16_t* best, std::uint16_t* best_cost, std::uint16_t* second, std::uint16_t* right_cost,
+             std::uint16_t* right_best, std::ptrdiff_t right_shift);
+
+Ranges ranges_from_coarse(const ImgF& coarse, int width, int height, const ImgU8& fine_valid,
+                          const ImgU8& partner_valid, int partner_width, int partner_sign, int dmin, int nd,
+                          const r3dg_options& o, bool mirrored);
+
+void remove_speckles(ImgF& d, int min_size, float max_diff);
+ImgF median3x3(const ImgF& d);
+
+struct MatchOutput {
+  ImgF disparity;
+  ImgF confidence;
+  Img<std::int16_t> labels;  // level 0: dominant tile's integer disparity (INT16_MIN: none)
+  ImgU8 lr_mask;             // level 0: 1 where the LR check removed a valid pixel
+  int disparity_min = 0;
+  int num_disparities = 0;
+  int levels = 0;
+  int tiles_level0 = 0;
+  std::uint64_t cells = 0;
+  std::uint64_t tile_pixels = 0;
+};
+
+MatchOutput match_stereo(const ImgU16& left, const ImgU16& right, const ImgU8& left_valid, const ImgU8& right_valid,
+                         int dmin, int nd, const r3dg_options& o, int threads);
+
+void check_options(const r3dg_options& o);
+
+}  // namespace oracle

@@ -1,0 +1,106 @@
+"""ctypes mirror of the POD types in include/r3dg.h (C ABI of the GPU matcher).
+
+Pure data definitions: no library is loaded here.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import enum
+
+R3DG_HOST = 0
+R3DG_DEVICE = 1
+
+
+class Status(enum.IntEnum):
+    OK = 0
+    INVALID_ARGUMENT = 1
+    OUT_OF_MEMORY = 2
+    DEVICE = 3
+    CANCELLED = 4
+    BUDGET = 5
+    NOT_FOUND = 6
+    INTERNAL = 7
+
+
+class SubpixelMethod(enum.IntEnum):
+    """r3d::SubpixelMethod (stereo_matcher.hpp:23-27)."""
+    none = 0
+    parabola = 1
+    equiangular = 2
+
+
+class CImage(ctypes.Structure):
+    _fields_ = [("data", ctypes.c_void_p), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("stride", ctypes.c_int64), ("memory", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class COptions(ctypes.Structure):
+    _fields_ = [("p1", ctypes.c_int32), ("p2", ctypes.c_int32), ("p2_adapt", ctypes.c_int32),
+                ("invalid_cost", ctypes.c_int32), ("subpixel", ctypes.c_int32), ("uniqueness", ctypes.c_float),
+                ("lr_max_diff", ctypes.c_int32), ("lr_full_match", ctypes.c_int32),
+                ("speckle_size", ctypes.c_int32), ("speckle_diff", ctypes.c_float), ("median", ctypes.c_int32),
+                ("levels", ctypes.c_int32), ("range_radius", ctypes.c_int32), ("range_margin", ctypes.c_int32),
+                ("range_fill", ctypes.c_int32), ("tile_size", ctypes.c_int32), ("tile_margin", ctypes.c_int32),
+                ("tile_blend", ctypes.c_int32), ("memory_budget", ctypes.c_uint64)]
+
+
+class CMatchInfo(ctypes.Structure):
+    _fields_ = [("disparity_min", ctypes.c_int32), ("num_disparities", ctypes.c_int32),
+                ("levels", ctypes.c_int32), ("tiles_level0", ctypes.c_int32), ("cells", ctypes.c_uint64),
+                ("tile_pixels", ctypes.c_uint64), ("seconds", ctypes.c_double)]
+
+
+class CMatchDebug(ctypes.Structure):
+    _fields_ = [("labels", CImage), ("lr_mask", CImage)]
+
+
+@dataclasses.dataclass
+class StereoOptions:
+    """r3d::StereoOptions (stereo_matcher.hpp:30-57) with the same defaults."""
+    p1: int = 28
+    p2: int = 200
+    p2_adapt: int = 4
+    invalid_cost: int = 31
+    subpixel: SubpixelMethod = SubpixelMethod.equiangular
+    uniqueness: float = 0.0
+    lr_max_diff: int = 1
+    lr_full_match: bool = True
+    speckle_size: int = 100
+    speckle_diff: float = 1.0
+    median: bool = True
+    levels: int = -1
+    range_radius: int = 3
+    range_margin: int = 4
+    range_fill: int = 64
+    tile_size: int = 512
+    tile_margin: int = 32
+    tile_blend: int = 32
+    memory_budget: int = 2 << 30
+
+    def to_c(self) -> COptions:
+        return COptions(int(self.p1), int(self.p2), int(self.p2_adapt), int(self.invalid_cost), int(self.subpixel),
+                        float(self.uniqueness), int(self.lr_max_diff), int(bool(self.lr_full_match)),
+                        int(self.speckle_size), float(self.speckle_diff), int(bool(self.median)), int(self.levels),
+                        int(self.range_radius), int(self.range_margin), int(self.range_fill), int(self.tile_size),
+                        int(self.tile_margin), int(self.tile_blend), int(self.memory_budget))
+
+
+@dataclasses.dataclass
+class DisparityMap:
+    """r3d::DisparityMap (stereo_matcher.hpp:61-68) plus the level-0 integer outputs."""
+    disparity: object
+    confidence: object
+    disparity_min: int
+    num_disparities: int
+    levels: int
+    cells: int
+    tile_pixels: int = 0
+    tiles_level0: int = 0
+    labels: object = None
+    lr_mask: object = None
+    seconds: float = 0.0
+
+
+def align16(n: int) -> int:
+    return (n + 15) // 16 * 16
