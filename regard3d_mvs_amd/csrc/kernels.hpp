@@ -1,0 +1,77 @@
+// Host-side launchers of the gfx950 kernels (kernels_image.hip, kernels_sgm.hip).
+// Every launcher enqueues on `s` and returns without synchronising.  Pointers
+// are device pointers; strides are in elements.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "sgm_types.hpp"
+
+namespace r3dg {
+
+// ---- images (kernels_image.hip)
+void launch_census(const std::uint16_t* img, std::int64_t is, int w, int h, std::uint64_t* out, std::int64_t os,
+                   hipStream_t s);
+void launch_pyramid_u16(const std::uint16_t* in, std::int64_t is, int w, int h, std::uint32_t* tmp,
+                        std::uint16_t* out, std::int64_t os, hipStream_t s);
+void launch_pyramid_u8(const std::uint8_t* in, std::int64_t is, int w, int h, std::uint32_t* tmp, std::uint8_t* out,
+                       std::int64_t os, bool threshold, hipStream_t s);
+void launch_mirror_u16(const std::uint16_t* in, std::int64_t is, int w, int h, std::uint16_t* out, std::int64_t os,
+                       hipStream_t s);
+void launch_mirror_u8(const std::uint8_t* in, std::int64_t is, int w, int h, std::uint8_t* out, std::int64_t os,
+                      hipStream_t s);
+// rows[y] = {first, last} valid column (mask >= 128); mask == nullptr: {0, w - 1}.
+void launch_valid_intervals(const std::uint8_t* mask, std::int64_t ms, int w, int h, std::int32_t* rows,
+                            hipStream_t s);
+void launch_fill_f32(float* p, std::size_t n, float v, hipStream_t s);
+void launch_fill_u8(std::uint8_t* p, std::size_t n, std::uint8_t v, hipStream_t s);
+
+// Left-right check (stereo_matcher.cpp:777-805, 842-880).
+// right(x) = mirrored(wr - 1 - x) - (wr - wl)
+void launch_unmirror_right(const float* mirrored, int wr, int h, int wl, float* right, hipStream_t s);
+// bad[x] = map valid and left_right_differ(map, other, sign) (1 / 0); with
+// `apply`, invalidates map[x] too.  Reads `other` only.
+void launch_lr_check(float* map, int w, int h, const float* other, int wo, int sign, int max_diff,
+                     std::uint8_t* bad, bool apply, hipStream_t s);
+void launch_apply_mask(float* map, std::size_t n, const std::uint8_t* bad, hipStream_t s);
+
+// Post-filters (stereo_matcher.cpp:919-1021).  d is contiguous w x h.
+void launch_speckles(float* d, int w, int h, int min_size, float max_diff, std::int32_t* parent, std::int32_t* size,
+                     hipStream_t s);
+void launch_median3x3(const float* in, int w, int h, float* out, hipStream_t s);
+
+// ranges_from_coarse (stereo_matcher.cpp:1124-1267).  scratch needs
+// ranges_scratch_floats(wc, hc) floats.
+std::size_t ranges_scratch_floats(int wc, int hc, int range_fill);
+void launch_ranges_from_coarse(const float* coarse, int wc, int hc, int width, int height, const std::uint8_t* fine_valid,
+                               std::int64_t fvs, const std::int32_t* partner_rows, int dmin, int n_all, int radius,
+                               int margin, int fill, int partner_sign, bool mirrored, float* scratch,
+                               std::uint16_t* lo, std::uint16_t* num, std::int64_t rs, hipStream_t s);
+
+// ---- aggregation (kernels_sgm.hip)
+// Cells of every tile row (row_cells[row_base + y]) and pixel offsets inside
+// the row (col_off); one block per tile row.
+// row_max = largest label count of the row.
+void launch_tile_row_scan(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* row_prefix, int ntiles,
+                          std::uint64_t total_rows, std::uint32_t* col_off, std::uint64_t* row_cells,
+                          std::uint32_t* row_max, hipStream_t s);
+// Exclusive scan of row_cells per tile into row_start; tile_cells[t] = total,
+// tile_max[t] = largest label count of the tile.
+void launch_tile_cells(const TileDesc* tiles, int ntiles, const std::uint64_t* row_cells, const std::uint32_t* row_max,
+                       std::uint64_t* row_start, std::uint64_t* tile_cells, std::uint32_t* tile_max, hipStream_t s);
+// One SGM path over every tile (path index 0..7, see kernels_sgm.hip).
+void launch_sgm_path(int path, const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* group_prefix, int ntiles,
+                     std::uint64_t groups, const std::uint64_t* row_start, const std::uint32_t* col_off,
+                     std::uint16_t* S, bool accumulate, int max_num, hipStream_t s);
+// WTA + sub-pixel + checks of every tile row into per-tile-pixel results.
+void launch_select(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* row_prefix, int ntiles,
+                   std::uint64_t total_rows, const std::uint64_t* row_start, const std::uint32_t* col_off,
+                   const std::uint16_t* S, SelectParams p, int max_w, int max_D, float* tdisp, float* tconf,
+                   std::int16_t* tlab, hipStream_t s);
+// Blends the tiles of one job into a full-size map.
+void launch_blend(GridDesc g, const TileDesc* tiles, const float* tdisp, const float* tconf, const std::int16_t* tlab,
+                  float* disp, float* conf, std::int16_t* lab, hipStream_t s);
+
+}  // namespace r3dg

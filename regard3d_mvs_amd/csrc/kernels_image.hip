@@ -1,0 +1,531 @@
+// gfx950 kernels of the per-pixel stages around the aggregation: Census,
+// pyramid, masks, left-right check, speckle / median filters and the
+// coarse-to-fine label ranges.  All are HBM-bound integer or exact float work
+// with one thread per output pixel and row-coalesced accesses.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace r3dg {
+namespace {
+
+constexpr float kInf = __builtin_huge_valf();
+
+// std::min / std::max of the reference (first argument wins ties).
+__device__ __forceinline__ float smin(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float smax(float a, float b) { return a < b ? b : a; }
+__device__ __forceinline__ bool finite(float v) { return __builtin_isfinite(v); }
+
+inline dim3 grid2(int w, int h, dim3 b) { return dim3((w + b.x - 1) / b.x, (h + b.y - 1) / b.y); }
+
+// ------------------------------------------------------------------ Census
+// 9x7 window (census.cpp:20-35, 104-113): raster-order neighbours, centre
+// skipped, bit 61 - k set when neighbour k < centre; rows clamped, columns
+// replicated (census.cpp:137-155).  A 64x4 block stages its (64+8)x(4+6) u16
+// halo in LDS with coalesced row loads and writes one u64 per thread.
+constexpr int kCw = 64, kCh = 4;
+__global__ __launch_bounds__(256) void k_census(const std::uint16_t* __restrict__ img, std::int64_t is, int w, int h,
+                                                std::uint64_t* __restrict__ out, std::int64_t os) {
+  __shared__ std::uint16_t t[kCh + 6][kCw + 8];
+  const int bx = blockIdx.x * kCw, by = blockIdx.y * kCh;
+  const int tid = threadIdx.y * kCw + threadIdx.x;
+  for (int i = tid; i < (kCh + 6) * (kCw + 8); i += 256) {
+    const int ly = i / (kCw + 8), lx = i - ly * (kCw + 8);
+    const int gy = min(max(by + ly - 3, 0), h - 1);
+    const int gx = min(max(bx + lx - 4, 0), w - 1);
+    t[ly][lx] = img[gy * is + gx];
+  }
+  __syncthreads();
+  const int x = bx + threadIdx.x, y = by + threadIdx.y;
+  if (x >= w || y >= h) return;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const std::uint16_t c = t[ty + 3][tx + 4];
+  std::uint32_t hi = 0, lo = 0;  // bits 61..32 (30 neighbours), 31..0 (32 neighbours)
+  int k = 0;
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      if (j == 3 && i == 4) continue;
+      const std::uint32_t b = t[ty + j][tx + i] < c ? 1U : 0U;
+      if (k < 30) {
+        hi = (hi << 1) | b;
+      } else {
+        lo = (lo << 1) | b;
+      }
+      ++k;
+    }
+  }
+  out[y * os + x] = (std::uint64_t(hi) << 32) | lo;
+}
+
+// ------------------------------------------------------------------ pyramid
+// pyramid.cpp:34-85: [1 4 6 4 1] horizontally into uint32, then vertically,
+// (sum + 128) >> 8, replicated borders.
+template <typename T>
+__global__ void k_pyr_h(const T* __restrict__ in, std::int64_t is, int w, int h, int dw,
+                        std::uint32_t* __restrict__ tmp) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= dw) return;
+  const T* r = in + y * is;
+  const int sx = 2 * x;
+  const std::uint32_t a = r[max(sx - 2, 0)], b = r[max(sx - 1, 0)], c = r[sx], d = r[min(sx + 1, w - 1)],
+                      e = r[min(sx + 2, w - 1)];
+  tmp[std::int64_t(y) * dw + x] = a + e + 4U * (b + d) + 6U * c;
+}
+
+template <typename T>
+__global__ void k_pyr_v(const std::uint32_t* __restrict__ tmp, int h, int dw, int dh, T* __restrict__ out,
+                        std::int64_t os, int threshold) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= dw) return;
+  const int sy = 2 * y;
+  auto at = [&](int yy) { return tmp[std::int64_t(yy) * dw + x]; };
+  const std::uint32_t s = at(max(sy - 2, 0)) + at(min(sy + 2, h - 1)) + 4U * (at(max(sy - 1, 0)) + at(min(sy + 1, h - 1))) +
+                          6U * at(sy);
+  std::uint32_t v = (s + 128U) >> 8U;
+  if (threshold) v = v >= 128U ? 255U : 0U;  // reduce_mask, stereo_matcher.cpp:892-900
+  out[y * os + x] = static_cast<T>(v);
+}
+
+template <typename T>
+__global__ void k_mirror(const T* __restrict__ in, std::int64_t is, int w, int h, T* __restrict__ out,
+                         std::int64_t os) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= w) return;
+  out[y * os + x] = in[y * is + (w - 1 - x)];
+}
+
+// valid_intervals (stereo_matcher.cpp:43-67): first and last column with
+// mask >= 128; {0, -1} for a row without one.
+__global__ __launch_bounds__(256) void k_valid_intervals(const std::uint8_t* __restrict__ mask, std::int64_t ms, int w,
+                                                         std::int32_t* __restrict__ rows) {
+  const int y = blockIdx.x;
+  int first = w, last = -1;
+  if (mask == nullptr) {
+    first = 0;
+    last = w - 1;
+  } else {
+    const std::uint8_t* m = mask + y * ms;
+    for (int x = threadIdx.x; x < w; x += 256) {
+      if (m[x] >= 128) {
+        first = min(first, x);
+        last = max(last, x);
+      }
+    }
+  }
+  __shared__ int sf[256], sl[256];
+  sf[threadIdx.x] = first;
+  sl[threadIdx.x] = last;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) {
+      sf[threadIdx.x] = min(sf[threadIdx.x], sf[threadIdx.x + k]);
+      sl[threadIdx.x] = max(sl[threadIdx.x], sl[threadIdx.x + k]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const bool any = sl[0] >= 0;
+    rows[2 * y] = any ? sf[0] : 0;
+    rows[2 * y + 1] = any ? sl[0] : -1;
+  }
+}
+
+template <typename T>
+__global__ void k_fill(T* p, std::size_t n, T v) {
+  const std::size_t i = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------ left-right check
+__global__ void k_unmirror_right(const float* __restrict__ mirrored, int wr, int h, float off,
+                                 float* __restrict__ right) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= wr) return;
+  right[std::int64_t(y) * wr + x] = mirrored[std::int64_t(y) * wr + (wr - 1 - x)] - off;
+}
+
+// left_right_differ (stereo_matcher.cpp:777-782): other column
+// lround(x + sign * d); invalid if outside, not finite or |d - d_other| > max_diff.
+__global__ void k_lr_check(float* __restrict__ map, int w, int h, const float* __restrict__ other, int wo, float sign,
+                           float max_diff, std::uint8_t* __restrict__ bad, int apply) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= w) return;
+  const std::int64_t i = std::int64_t(y) * w + x;
+  const float d = map[i];
+  bool differ = false;
+  if (finite(d)) {
+    const float pos = static_cast<float>(x) + sign * d;
+    const int xo = static_cast<int>(roundf(pos));  // lround: halves away from zero
+    const float dother = xo >= 0 && xo < wo ? other[std::int64_t(y) * wo + xo] : kInf;
+    differ = !finite(dother) || fabsf(d - dother) > max_diff;
+  }
+  if (bad != nullptr) bad[i] = differ ? 1 : 0;
+  if (apply && differ) map[i] = kInf;
+}
+
+__global__ void k_apply_mask(float* __restrict__ map, std::size_t n, const std::uint8_t* __restrict__ bad) {
+  const std::size_t i = std::size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n && bad[i] != 0) map[i] = kInf;
+}
+
+// ------------------------------------------------------------------ speckle filter
+// remove_speckles (stereo_matcher.cpp:919-970) is a BFS whose segments are the
+// connected components of the 4-neighbour graph with edges
+// |d_j - d_i| <= max_diff (the relation is symmetric, so the segments do not
+// depend on the visiting order).  Here: lock-free union-find with atomicMin
+// (Playne & Hawick 2018), then component sizes, then invalidation.
+__device__ __forceinline__ int cc_find(int* p, int i) {
+  int v = __hip_atomic_load(&p[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (v != i) {
+    i = v;
+    v = __hip_atomic_load(&p[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return i;
+}
+
+__device__ void cc_union(int* p, int a, int b) {
+  for (;;) {
+    a = cc_find(p, a);
+    b = cc_find(p, b);
+    if (a == b) return;
+    if (a > b) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    const int old = atomicMin(&p[b], a);
+    if (old == b) return;
+    b = old;
+  }
+}
+
+__global__ void k_cc_init(int* __restrict__ p, int* __restrict__ size, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    p[i] = i;
+    size[i] = 0;
+  }
+}
+
+__global__ void k_cc_union(const float* __restrict__ d, int w, int h, float max_diff, int* p) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= w) return;
+  const int i = y * w + x;
+  const float v = d[i];
+  if (x + 1 < w && fabsf(d[i + 1] - v) <= max_diff) cc_union(p, i, i + 1);
+  if (y + 1 < h && fabsf(d[i + w] - v) <= max_diff) cc_union(p, i, i + w);
+}
+
+__global__ void k_cc_count(int* p, int* __restrict__ size, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int r = cc_find(p, i);
+  p[i] = r;
+  atomicAdd(&size[r], 1);
+}
+
+__global__ void k_cc_apply(float* __restrict__ d, const int* __restrict__ p, const int* __restrict__ size, int n,
+                           int min_size) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && size[p[i]] < min_size) d[i] = kInf;
+}
+
+// ------------------------------------------------------------------ median
+// median_filter_3x3 (stereo_matcher.cpp:972-1021): Paeth's median-of-9
+// network for an all-valid interior window (median9, :903-915), else the median
+// of the valid values (mean of the two middle ones for an even count).
+__global__ void k_median3x3(const float* __restrict__ in, int w, int h, float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= w) return;
+  const float c = in[std::int64_t(y) * w + x];
+  if (!finite(c)) {
+    out[std::int64_t(y) * w + x] = c;
+    return;
+  }
+  float p[9];
+  int n = 0;
+  bool all = x > 0 && y > 0 && x + 1 < w && y + 1 < h;
+  for (int dy = -1; dy <= 1; ++dy) {
+    for (int dx = -1; dx <= 1; ++dx) {
+      const int qx = x + dx, qy = y + dy;
+      if (qx >= 0 && qy >= 0 && qx < w && qy < h) {
+        const float v = in[std::int64_t(qy) * w + qx];
+        if (finite(v)) {
+          p[n++] = v;
+        } else {
+          all = false;
+        }
+      }
+    }
+  }
+  float m;
+  if (all) {
+    auto s2 = [&](int i, int j) {
+      const float a = p[i], b = p[j];
+      p[i] = smin(a, b);
+      p[j] = smax(a, b);
+    };
+    s2(1, 2); s2(4, 5); s2(7, 8); s2(0, 1); s2(3, 4); s2(6, 7);
+    s2(1, 2); s2(4, 5); s2(7, 8); s2(0, 3); s2(5, 8); s2(4, 7);
+    s2(3, 6); s2(1, 4); s2(2, 5); s2(4, 7); s2(4, 2); s2(6, 4);
+    s2(4, 2);
+    m = p[4];
+  } else {
+    for (int i = 1; i < n; ++i) {  // insertion sort of the valid values
+      const float v = p[i];
+      int j = i - 1;
+      while (j >= 0 && v < p[j]) {
+        p[j + 1] = p[j];
+        --j;
+      }
+      p[j + 1] = v;
+    }
+    m = p[n / 2];
+    if (n % 2 == 0) m = 0.5F * (m + p[n / 2 - 1]);
+  }
+  out[std::int64_t(y) * w + x] = m;
+}
+
+// ------------------------------------------------------------------ label ranges
+// ranges_from_coarse (stereo_matcher.cpp:1124-1267).
+__global__ void k_rng_h(const float* __restrict__ coarse, int wc, int hc, int r, float* __restrict__ hmin,
+                        float* __restrict__ hmax) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= wc) return;
+  float mn = kInf, mx = -kInf;
+  for (int k = max(0, x - r); k <= min(wc - 1, x + r); ++k) {
+    const float v = coarse[std::int64_t(y) * wc + k];
+    if (finite(v)) {
+      mn = smin(mn, v);
+      mx = smax(mx, v);
+    }
+  }
+  hmin[std::int64_t(y) * wc + x] = mn;
+  hmax[std::int64_t(y) * wc + x] = mx;
+}
+
+__global__ void k_rng_v(const float* __restrict__ hmin, const float* __restrict__ hmax, int wc, int hc, int r,
+                        float* __restrict__ cmin, float* __restrict__ cmax) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= wc) return;
+  float mn = kInf, mx = -kInf;
+  for (int k = max(0, y - r); k <= min(hc - 1, y + r); ++k) {
+    mn = smin(mn, hmin[std::int64_t(k) * wc + x]);
+    mx = smax(mx, hmax[std::int64_t(k) * wc + x]);
+  }
+  cmin[std::int64_t(y) * wc + x] = mn;
+  cmax[std::int64_t(y) * wc + x] = mx;
+}
+
+__global__ void k_rng_pyr(const float* __restrict__ amin, const float* __restrict__ amax, int aw, int ah, int nw,
+                          int nh, float* __restrict__ bmin, float* __restrict__ bmax) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= nw) return;
+  const int x0 = 2 * x, y0 = 2 * y, x1 = min(2 * x + 1, aw - 1), y1 = min(2 * y + 1, ah - 1);
+  auto A = [&](const float* a, int xx, int yy) { return a[std::int64_t(yy) * aw + xx]; };
+  bmin[std::int64_t(y) * nw + x] =
+      smin(smin(A(amin, x0, y0), A(amin, x1, y0)), smin(A(amin, x0, y1), A(amin, x1, y1)));
+  bmax[std::int64_t(y) * nw + x] =
+      smax(smax(A(amax, x0, y0), A(amax, x1, y0)), smax(A(amax, x0, y1), A(amax, x1, y1)));
+}
+
+struct RangePyr {
+  const float* mn[kMaxLevels];
+  const float* mx[kMaxLevels];
+  int w[kMaxLevels];
+  int h[kMaxLevels];
+  int levels;
+};
+
+// Pixels without a valid coarse disparity take the min / max of the first
+// pyramid level with a valid value in the 3x3 block neighbourhood.
+__global__ void k_rng_fill(float* __restrict__ cmin, float* __restrict__ cmax, int wc, int hc, RangePyr pyr) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= wc) return;
+  const std::int64_t i = std::int64_t(y) * wc + x;
+  if (cmin[i] != kInf) return;
+  for (int k = 1; k < pyr.levels; ++k) {
+    const int bx = x >> k, by = y >> k, aw = pyr.w[k], ah = pyr.h[k];
+    float mn = kInf, mx = -kInf;
+    for (int qy = max(0, by - 1); qy <= min(ah - 1, by + 1); ++qy) {
+      for (int qx = max(0, bx - 1); qx <= min(aw - 1, bx + 1); ++qx) {
+        mn = smin(mn, pyr.mn[k][std::int64_t(qy) * aw + qx]);
+        mx = smax(mx, pyr.mx[k][std::int64_t(qy) * aw + qx]);
+      }
+    }
+    if (mn != kInf) {
+      cmin[i] = mn;
+      cmax[i] = mx;
+      return;
+    }
+  }
+}
+
+__global__ void k_rng_final(const float* __restrict__ cmin, const float* __restrict__ cmax, int wc, int hc, int width,
+                            int height, const std::uint8_t* __restrict__ fine_valid, std::int64_t fvs,
+                            const std::int32_t* __restrict__ prow, int dmin, int n_all, int margin, int partner_sign,
+                            int mirrored, std::uint16_t* __restrict__ lo_out, std::uint16_t* __restrict__ num_out,
+                            std::int64_t rs) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= width) return;
+  const int pf = prow[2 * y], pl = prow[2 * y + 1];
+  const int cy0 = y / 2, cy1 = min(cy0 + 1, hc - 1), cx0 = x / 2, cx1 = min(cx0 + 1, wc - 1);
+  auto C = [&](const float* a, int xx, int yy) { return a[std::int64_t(yy) * wc + xx]; };
+  const float mn = smin(smin(C(cmin, cx0, cy0), C(cmin, cx1, cy0)), smin(C(cmin, cx0, cy1), C(cmin, cx1, cy1)));
+  const float mx = smax(smax(C(cmax, cx0, cy0), C(cmax, cx1, cy0)), smax(C(cmax, cx0, cy1), C(cmax, cx1, cy1)));
+  int lo = 0, hi = n_all - 1;
+  if (mn != kInf) {
+    lo = static_cast<int>(floorf(2.0F * mn)) - dmin - margin;
+    hi = static_cast<int>(ceilf(2.0F * mx)) - dmin + margin;
+  }
+  const int match_lo = (partner_sign < 0 ? x - pl : pf - x) - dmin;
+  const int match_hi = (partner_sign < 0 ? x - pf : pl - x) - dmin;
+  lo = max(lo, match_lo);
+  hi = min(hi, match_hi);
+  if (lo > hi || pl < pf || (fine_valid != nullptr && fine_valid[y * fvs + x] < 128)) hi = lo;
+  lo = min(max(lo, 0), n_all - 1);
+  hi = min(max(hi, lo), n_all - 1);
+  const int n = min(n_all, (hi - lo + kLabelAlign) / kLabelAlign * kLabelAlign);
+  lo = min(max(lo - (n - (hi - lo + 1)) / 2, 0), n_all - n);
+  const int xo = mirrored ? width - 1 - x : x;
+  lo_out[y * rs + xo] = static_cast<std::uint16_t>(lo);
+  num_out[y * rs + xo] = static_cast<std::uint16_t>(n);
+}
+
+int range_levels(int wc, int hc, int fill, int* ws, int* hs) {
+  int n = 1;
+  ws[0] = wc;
+  hs[0] = hc;
+  while (n < kMaxLevels && (2 << n) <= 2 * fill && ws[n - 1] > 1) {
+    ws[n] = (ws[n - 1] + 1) / 2;
+    hs[n] = (hs[n - 1] + 1) / 2;
+    ++n;
+  }
+  return n;
+}
+
+}  // namespace
+
+void launch_census(const std::uint16_t* img, std::int64_t is, int w, int h, std::uint64_t* out, std::int64_t os,
+                   hipStream_t s) {
+  const dim3 b(kCw, kCh);
+  k_census<<<grid2(w, h, b), b, 0, s>>>(img, is, w, h, out, os);
+}
+
+void launch_pyramid_u16(const std::uint16_t* in, std::int64_t is, int w, int h, std::uint32_t* tmp,
+                        std::uint16_t* out, std::int64_t os, hipStream_t s) {
+  const int dw = (w + 1) / 2, dh = (h + 1) / 2;
+  k_pyr_h<std::uint16_t><<<dim3((dw + 255) / 256, h), 256, 0, s>>>(in, is, w, h, dw, tmp);
+  k_pyr_v<std::uint16_t><<<dim3((dw + 255) / 256, dh), 256, 0, s>>>(tmp, h, dw, dh, out, os, 0);
+}
+
+void launch_pyramid_u8(const std::uint8_t* in, std::int64_t is, int w, int h, std::uint32_t* tmp, std::uint8_t* out,
+                       std::int64_t os, bool threshold, hipStream_t s) {
+  const int dw = (w + 1) / 2, dh = (h + 1) / 2;
+  k_pyr_h<std::uint8_t><<<dim3((dw + 255) / 256, h), 256, 0, s>>>(in, is, w, h, dw, tmp);
+  k_pyr_v<std::uint8_t><<<dim3((dw + 255) / 256, dh), 256, 0, s>>>(tmp, h, dw, dh, out, os, threshold ? 1 : 0);
+}
+
+void launch_mirror_u16(const std::uint16_t* in, std::int64_t is, int w, int h, std::uint16_t* out, std::int64_t os,
+                       hipStream_t s) {
+  k_mirror<std::uint16_t><<<dim3((w + 255) / 256, h), 256, 0, s>>>(in, is, w, h, out, os);
+}
+
+void launch_mirror_u8(const std::uint8_t* in, std::int64_t is, int w, int h, std::uint8_t* out, std::int64_t os,
+                      hipStream_t s) {
+  k_mirror<std::uint8_t><<<dim3((w + 255) / 256, h), 256, 0, s>>>(in, is, w, h, out, os);
+}
+
+void launch_valid_intervals(const std::uint8_t* mask, std::int64_t ms, int w, int h, std::int32_t* rows,
+                            hipStream_t s) {
+  k_valid_intervals<<<h, 256, 0, s>>>(mask, ms, w, rows);
+}
+
+void launch_fill_f32(float* p, std::size_t n, float v, hipStream_t s) {
+  if (n == 0) return;
+  k_fill<float><<<(n + 255) / 256, 256, 0, s>>>(p, n, v);
+}
+
+void launch_fill_u8(std::uint8_t* p, std::size_t n, std::uint8_t v, hipStream_t s) {
+  if (n == 0) return;
+  k_fill<std::uint8_t><<<(n + 255) / 256, 256, 0, s>>>(p, n, v);
+}
+
+void launch_unmirror_right(const float* mirrored, int wr, int h, int wl, float* right, hipStream_t s) {
+  k_unmirror_right<<<dim3((wr + 255) / 256, h), 256, 0, s>>>(mirrored, wr, h, static_cast<float>(wr - wl), right);
+}
+
+void launch_lr_check(float* map, int w, int h, const float* other, int wo, int sign, int max_diff,
+                     std::uint8_t* bad, bool apply, hipStream_t s) {
+  k_lr_check<<<dim3((w + 255) / 256, h), 256, 0, s>>>(map, w, h, other, wo, static_cast<float>(sign),
+                                                     static_cast<float>(max_diff), bad, apply ? 1 : 0);
+}
+
+void launch_apply_mask(float* map, std::size_t n, const std::uint8_t* bad, hipStream_t s) {
+  if (n == 0) return;
+  k_apply_mask<<<(n + 255) / 256, 256, 0, s>>>(map, n, bad);
+}
+
+void launch_speckles(float* d, int w, int h, int min_size, float max_diff, std::int32_t* parent, std::int32_t* size,
+                     hipStream_t s) {
+  const int n = w * h;
+  if (min_size <= 1 || n == 0) return;
+  k_cc_init<<<(n + 255) / 256, 256, 0, s>>>(parent, size, n);
+  k_cc_union<<<dim3((w + 255) / 256, h), 256, 0, s>>>(d, w, h, max_diff, parent);
+  k_cc_count<<<(n + 255) / 256, 256, 0, s>>>(parent, size, n);
+  k_cc_apply<<<(n + 255) / 256, 256, 0, s>>>(d, parent, size, n, min_size);
+}
+
+void launch_median3x3(const float* in, int w, int h, float* out, hipStream_t s) {
+  k_median3x3<<<dim3((w + 255) / 256, h), 256, 0, s>>>(in, w, h, out);
+}
+
+std::size_t ranges_scratch_floats(int wc, int hc, int range_fill) {
+  int ws[kMaxLevels], hs[kMaxLevels];
+  const int n = range_levels(wc, hc, range_fill, ws, hs);
+  std::size_t total = 4 * std::size_t(wc) * hc;  // hmin, hmax, cmin, cmax
+  for (int k = 1; k < n; ++k) total += 2 * std::size_t(ws[k]) * hs[k];
+  return total;
+}
+
+void launch_ranges_from_coarse(const float* coarse, int wc, int hc, int width, int height, const std::uint8_t* fine_valid,
+                               std::int64_t fvs, const std::int32_t* partner_rows, int dmin, int n_all, int radius,
+                               int margin, int fill, int partner_sign, bool mirrored, float* scratch,
+                               std::uint16_t* lo, std::uint16_t* num, std::int64_t rs, hipStream_t s) {
+  const std::size_t a = std::size_t(wc) * hc;
+  float* hmin = scratch;
+  float* hmax = scratch + a;
+  float* cmin = scratch + 2 * a;
+  float* cmax = scratch + 3 * a;
+  const dim3 gc((wc + 255) / 256, hc);
+  k_rng_h<<<gc, 256, 0, s>>>(coarse, wc, hc, radius, hmin, hmax);
+  k_rng_v<<<gc, 256, 0, s>>>(hmin, hmax, wc, hc, radius, cmin, cmax);
+  RangePyr pyr{};
+  int ws[kMaxLevels], hs[kMaxLevels];
+  pyr.levels = range_levels(wc, hc, fill, ws, hs);
+  pyr.mn[0] = cmin;
+  pyr.mx[0] = cmax;
+  float* next = scratch + 4 * a;
+  for (int k = 0; k < pyr.levels; ++k) {
+    pyr.w[k] = ws[k];
+    pyr.h[k] = hs[k];
+    if (k == 0) continue;
+    float* bmin = next;
+    float* bmax = next + std::size_t(ws[k]) * hs[k];
+    next = bmax + std::size_t(ws[k]) * hs[k];
+    k_rng_pyr<<<dim3((ws[k] + 255) / 256, hs[k]), 256, 0, s>>>(pyr.mn[k - 1], pyr.mx[k - 1], ws[k - 1], hs[k - 1],
+                                                                ws[k], hs[k], bmin, bmax);
+    pyr.mn[k] = bmin;
+    pyr.mx[k] = bmax;
+  }
+  if (pyr.levels > 1) k_rng_fill<<<gc, 256, 0, s>>>(cmin, cmax, wc, hc, pyr);
+  k_rng_final<<<dim3((width + 255) / 256, height), 256, 0, s>>>(cmin, cmax, wc, hc, width, height, fine_valid, fvs,
+                                                                 partner_rows, dmin, n_all, margin, partner_sign,
+                                                                 mirrored ? 1 : 0, lo, num, rs);
+}
+
+}  // namespace r3dg

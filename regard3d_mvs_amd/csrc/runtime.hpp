@@ -1,0 +1,164 @@
+// Host runtime of the GPU matcher: context, device memory arena, errors.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+
+#include "r3dg.h"
+
+namespace r3dg {
+
+// Exception carrying an r3dg_status; converted to a return code at the C ABI.
+struct Failure : std::runtime_error {
+  int32_t status;
+  Failure(int32_t s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+[[noreturn]] inline void fail(int32_t status, const std::string& m) { throw Failure(status, m); }
+
+inline void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+      (void)hipGetLastError();
+      fail(R3DG_ERROR_OUT_OF_MEMORY, std::string(what) + ": " + hipGetErrorString(e));
+    }
+    fail(R3DG_ERROR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+// Caching device allocator bounded by the context's HBM budget.  Freed blocks
+// are kept for reuse (every matcher call allocates the same shapes again);
+// blocks are rounded to 64 KiB (small) or 2 MiB (large) so shapes of
+// different pairs share them.
+class Arena {
+ public:
+  explicit Arena(std::uint64_t budget) : budget_(budget) {}
+  Arena(const Arena&) = delete;
+  Arena& operator=(const Arena&) = delete;
+  ~Arena() { trim(); }
+
+  void* alloc(std::size_t bytes) {
+    const std::size_t size = round(bytes);
+    auto it = free_.lower_bound(size);
+    if (it != free_.end() && it->first <= 2 * size) {
+      void* p = it->second;
+      live_[p] = it->first;
+      free_.erase(it);
+      return p;
+    }
+    if (held_ + size > budget_) trim();
+    if (held_ + size > budget_) {
+      fail(R3DG_ERROR_BUDGET, "device memory budget exceeded: need " + std::to_string(size >> 20) + " MiB more, " +
+                                  std::to_string(held_ >> 20) + " of " + std::to_string(budget_ >> 20) +
+                                  " MiB held");
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, size);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      trim();
+      e = hipMalloc(&p, size);
+    }
+    hip_check(e, "hipMalloc");
+    held_ += size;
+    live_[p] = size;
+    return p;
+  }
+
+  void release(void* p) {
+    if (p == nullptr) return;
+    auto it = live_.find(p);
+    if (it == live_.end()) return;
+    free_.emplace(it->second, p);
+    live_.erase(it);
+  }
+
+  void trim() {
+    for (auto& [size, p] : free_) {
+      (void)hipFree(p);
+      held_ -= size;
+    }
+    free_.clear();
+  }
+
+  [[nodiscard]] std::uint64_t held() const { return held_; }
+  [[nodiscard]] std::uint64_t budget() const { return budget_; }
+
+ private:
+  static std::size_t round(std::size_t b) {
+    const std::size_t g = b >= (std::size_t{4} << 20) ? (std::size_t{2} << 20) : (std::size_t{64} << 10);
+    return (std::max<std::size_t>(b, 1) + g - 1) / g * g;
+  }
+  std::multimap<std::size_t, void*> free_;
+  std::unordered_map<void*, std::size_t> live_;
+  std::uint64_t held_ = 0;
+  std::uint64_t budget_;
+};
+
+// RAII device buffer of n elements of T from an arena.
+template <typename T>
+class DBuf {
+ public:
+  DBuf() = default;
+  DBuf(Arena& a, std::size_t n) : arena_(&a), n_(n) { p_ = static_cast<T*>(a.alloc(n * sizeof(T))); }
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : arena_(o.arena_), p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) {
+      reset();
+      arena_ = o.arena_;
+      p_ = o.p_;
+      n_ = o.n_;
+      o.p_ = nullptr;
+      o.n_ = 0;
+    }
+    return *this;
+  }
+  ~DBuf() { reset(); }
+  void reset() {
+    if (p_ != nullptr && arena_ != nullptr) arena_->release(p_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  [[nodiscard]] T* get() const { return p_; }
+  [[nodiscard]] std::size_t size() const { return n_; }
+
+ private:
+  Arena* arena_ = nullptr;
+  T* p_ = nullptr;
+  std::size_t n_ = 0;
+};
+
+// Device image: a strided view plus optional ownership.
+template <typename T>
+struct DImg {
+  T* p = nullptr;
+  int w = 0;
+  int h = 0;
+  std::int64_t s = 0;
+  DBuf<T> own;
+  [[nodiscard]] bool empty() const { return p == nullptr || w == 0 || h == 0; }
+  [[nodiscard]] std::size_t pixels() const { return std::size_t(w) * std::size_t(h); }
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Arena arena;
+  std::mutex mutex;
+  std::atomic<int> cancel{0};
+  std::string error;
+  int32_t error_status = R3DG_OK;
+  explicit Context(std::uint64_t budget) : arena(budget) {}
+};
+
+}  // namespace r3dg

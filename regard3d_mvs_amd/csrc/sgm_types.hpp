@@ -1,0 +1,76 @@
+// Descriptors shared by the host driver and the gfx950 kernels.
+//
+// A *job* is one matching direction of one pyramid level of one pair: a base
+// image matched against a partner image (the left-to-right match, or the
+// right-to-left match on the mirrored pair, stereo_matcher.cpp:842-863).  A
+// *tile* is one of the job's overlapping tiles (stereo_matcher.cpp:613-653);
+// aggregation restarts at tile borders, so tiles are independent work units
+// and one launch can cover the tiles of many jobs (both directions of a level,
+// or every pair of a batch).
+//
+// HBM layout of the aggregated costs S (uint16): every tile owns a contiguous
+// packed volume at cell_base; inside it, tile row y starts at
+// row_start[row_base + y] and pixel x of that row at col_off[px_base + y*w + x]
+// (the TileLabels layout of stereo_matcher.cpp:172-224).  Label counts are
+// multiples of 16, so every pixel's labels start on a 32-byte boundary.
+#pragma once
+
+#include <cstdint>
+
+namespace r3dg {
+
+constexpr int kLabelAlign = 16;
+constexpr int kMaxLevels = 13;
+
+struct JobDesc {
+  const std::uint16_t* img;  // base image (P2 intensities), width x height
+  const std::uint64_t* cl;   // base Census
+  const std::uint64_t* cr;   // partner Census, pwidth x height
+  const std::int32_t* lrows; // [height][2] valid interval of the base rows
+  const std::int32_t* rrows; // [height][2] valid interval of the partner rows
+  const std::uint16_t* lo;   // per-pixel label ranges (nullptr: full range)
+  const std::uint16_t* num;
+  std::int64_t img_stride;   // elements
+  std::int64_t cl_stride;
+  std::int64_t cr_stride;
+  std::int64_t r_stride;
+  std::int32_t width;
+  std::int32_t height;
+  std::int32_t pwidth;
+  std::int32_t dmin;
+  std::int32_t D;            // aligned label count of the level
+  std::int32_t p1;
+  std::int32_t invalid_cost;
+  std::int32_t confidence;   // select writes confidence
+  std::uint16_t lut[256];    // P2 by 8-bit intensity step (stereo_matcher.cpp:69-79)
+};
+
+struct TileDesc {
+  std::int32_t x0, y0, w, h;  // in the job's base image
+  std::int32_t job;
+  std::int32_t max_num;       // largest label count of the tile
+  std::uint64_t cell_base;    // first cell of the tile in S
+  std::uint64_t px_base;      // first tile pixel in per-pixel arrays
+  std::uint64_t row_base;     // first tile row in row_start
+};
+
+// Tile grid of one job for blending (stereo_matcher.cpp:688-766): tile (i, j)
+// of the grid is tile tile_base + j * nx + i; cols[i] / rows[j] hold the tile
+// extent [x0, x1) and its core [c0, c1).
+struct GridDesc {
+  const std::int32_t* cols;  // [nx][4]
+  const std::int32_t* rows;  // [ny][4]
+  std::int32_t nx, ny;
+  std::int32_t width, height;
+  std::int32_t blend;
+  std::int32_t tile_base;
+};
+
+struct SelectParams {
+  std::int32_t subpixel;     // r3dg_subpixel
+  float uniqueness;
+  std::int32_t lr_max_diff;
+  std::int32_t diagonal_lr;  // check against the right WTA of the same volume
+};
+
+}  // namespace r3dg

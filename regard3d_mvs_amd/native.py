@@ -1,0 +1,251 @@
+"""Python binding of libr3dg.so (the C ABI in include/r3dg.h).
+
+Accepts numpy arrays (host memory) or torch tensors on the GPU (device
+memory, used in place).  Loading fails loudly when the library is missing:
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from ._abi import (R3DG_DEVICE, R3DG_HOST, CImage, CMatchDebug, CMatchInfo, DisparityMap, Status, StereoOptions,
+                   align16)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libr3dg.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "r3dg.h")
+
+_lib = None
+
+
+class R3dgError(RuntimeError):
+    """r3d::Error analogue: a non-zero r3dg_status."""
+
+    def __init__(self, status, message):
+        super().__init__(f"{Status(status).name}: {message}")
+        self.status = Status(status)
+
+
+class Cancelled(R3dgError):
+    pass
+
+
+def declared_symbols() -> list[str]:
+    """Every function declared in include/r3dg.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(r3dg_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libr3dg.so is not built ({LIB_PATH}); run `python -m regard3d_mvs_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name in declared_symbols():
+            fn = getattr(L, name)
+            fn.restype = ctypes.c_int32
+        L.r3dg_default_options.restype = None
+        _lib = L
+    return _lib
+
+
+def _raise(ctx, status):
+    buf = ctypes.create_string_buffer(4096)
+    lib().r3dg_last_error(ctx, buf, ctypes.c_size_t(len(buf)))
+    msg = buf.value.decode(errors="replace")
+    if status == Status.CANCELLED:
+        raise Cancelled(status, msg)
+    raise R3dgError(status, msg)
+
+
+def _is_torch(a):
+    return type(a).__module__.startswith("torch")
+
+
+def _image(a, dtype) -> tuple[CImage, object]:
+    """CImage view of a 2-D array; returns (view, keep-alive)."""
+    if a is None:
+        return CImage(None, 0, 0, 0, R3DG_HOST, 0), None
+    if _is_torch(a):
+        import torch
+        want = {np.uint8: torch.uint8, np.uint16: torch.uint16, np.float32: torch.float32,
+                np.int16: torch.int16, np.uint64: torch.uint64}[dtype]
+        if a.dtype != want:
+            raise TypeError(f"expected {want}, got {a.dtype}")
+        if a.dim() != 2 or a.stride(1) != 1:
+            raise ValueError("expected a 2-D tensor with unit column stride")
+        mem = R3DG_DEVICE if a.is_cuda else R3DG_HOST
+        return CImage(ctypes.c_void_p(a.data_ptr()), a.shape[1], a.shape[0], a.stride(0), mem, 0), a
+    arr = np.asarray(a)
+    if arr.dtype != np.dtype(dtype):
+        arr = arr.astype(dtype)
+    if arr.ndim != 2:
+        raise ValueError("expected a 2-D array")
+    if arr.strides[1] != arr.itemsize or arr.strides[0] % arr.itemsize != 0:
+        arr = np.ascontiguousarray(arr)
+    return CImage(arr.ctypes.data_as(ctypes.c_void_p), arr.shape[1], arr.shape[0], arr.strides[0] // arr.itemsize,
+                  R3DG_HOST, 0), arr
+
+
+def _like(ref, shape, dtype):
+    """Output buffer in the same memory as `ref` (torch device tensor or numpy)."""
+    if ref is not None and _is_torch(ref) and ref.is_cuda:
+        import torch
+        tdt = {np.uint8: torch.uint8, np.uint16: torch.uint16, np.float32: torch.float32, np.int16: torch.int16,
+               np.uint64: torch.uint64}[dtype]
+        return torch.empty(shape, dtype=tdt, device=ref.device)
+    return np.empty(shape, dtype)
+
+
+def _shape(a):
+    return (int(a.shape[0]), int(a.shape[1]))
+
+
+class Matcher:
+    """One context on one GPU (r3dg_ctx); the methods mirror the reference API."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self._ctx = ctypes.c_void_p()
+        rc = lib().r3dg_ctx_create(int(device), ctypes.c_uint64(hbm_budget), ctypes.byref(self._ctx))
+        if rc != 0:
+            buf = ctypes.create_string_buffer(4096)
+            lib().r3dg_global_error(buf, ctypes.c_size_t(len(buf)))
+            raise R3dgError(rc, buf.value.decode(errors="replace"))
+
+    def close(self):
+        if self._ctx:
+            lib().r3dg_ctx_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc != 0:
+            _raise(self._ctx, rc)
+
+    def cancel(self, requested=True):
+        self._check(lib().r3dg_cancel(self._ctx, int(bool(requested))))
+
+    # census_transform (census.hpp:20)
+    def census(self, img):
+        v, keep = _image(img, np.uint16)
+        out = _like(img, _shape(img), np.uint64)
+        o, keep2 = _image(out, np.uint64)
+        self._check(lib().r3dg_census(self._ctx, v, o))
+        return out
+
+    # pyramid_down (pyramid.hpp:20)
+    def pyramid_down(self, img):
+        dt = np.uint8 if str(img.dtype).endswith("uint8") else np.uint16
+        h, w = _shape(img)
+        v, keep = _image(img, dt)
+        out = _like(img, ((h + 1) // 2, (w + 1) // 2), dt)
+        o, keep2 = _image(out, dt)
+        fn = lib().r3dg_pyramid_down_u8 if dt == np.uint8 else lib().r3dg_pyramid_down_u16
+        self._check(fn(self._ctx, v, o))
+        return out
+
+    # detail::sgm_cost_volume (stereo_matcher.hpp:98-101)
+    def cost_volume(self, left, right, dmin, nd, options=None, ranges=None):
+        o = (options or StereoOptions()).to_c()
+        lv, k1 = _image(left, np.uint16)
+        rv, k2 = _image(right, np.uint16)
+        h, w = _shape(left)
+        if ranges is not None:
+            lo, num = ranges
+            lov, k3 = _image(lo, np.uint16)
+            nv, k4 = _image(num, np.uint16)
+            cells = int(np.asarray(num).astype(np.int64).sum())
+        else:
+            lov, k3 = _image(None, np.uint16)
+            nv, k4 = _image(None, np.uint16)
+            cells = w * h * align16(int(nd))
+        out = np.empty(cells, np.uint16)
+        self._check(lib().r3dg_cost_volume(self._ctx, lv, rv, int(dmin), int(nd), ctypes.byref(o), lov, nv,
+                                           out.ctypes.data_as(ctypes.c_void_p), ctypes.c_uint64(cells), R3DG_HOST))
+        return out
+
+    # match_stereo (stereo_matcher.hpp:82-85)
+    def match(self, left, right, left_valid=None, right_valid=None, dmin=0, nd=64, options=None, debug=False,
+              out=None):
+        o = (options or StereoOptions()).to_c()
+        h, w = _shape(left)
+        lv, k1 = _image(left, np.uint16)
+        rv, k2 = _image(right, np.uint16)
+        mlv, k3 = _image(left_valid, np.uint8)
+        mrv, k4 = _image(right_valid, np.uint8)
+        if out is not None:
+            disp, conf = out
+        else:
+            disp = _like(left, (h, w), np.float32)
+            conf = _like(left, (h, w), np.float32)
+        dv, k5 = _image(disp, np.float32)
+        cv, k6 = _image(conf, np.float32)
+        dbg = None
+        labels = lr = None
+        if debug:
+            labels = _like(left, (h, w), np.int16)
+            lr = _like(left, (h, w), np.uint8)
+            lab_v, k7 = _image(labels, np.int16)
+            lr_v, k8 = _image(lr, np.uint8)
+            dbg = CMatchDebug(lab_v, lr_v)
+        info = CMatchInfo()
+        self._check(lib().r3dg_match(self._ctx, lv, rv, mlv, mrv, int(dmin), int(nd), ctypes.byref(o), dv, cv,
+                                     ctypes.byref(dbg) if dbg is not None else None, ctypes.byref(info)))
+        return DisparityMap(disp, conf, info.disparity_min, info.num_disparities, info.levels, int(info.cells),
+                            int(info.tile_pixels), info.tiles_level0, labels, lr, info.seconds)
+
+    # remove_speckles (stereo_matcher.hpp:89): returns a filtered copy
+    def remove_speckles(self, d, min_size, max_diff):
+        d = np.array(d, dtype=np.float32, copy=True, order="C")
+        v, keep = _image(d, np.float32)
+        self._check(lib().r3dg_remove_speckles(self._ctx, v, int(min_size), ctypes.c_float(max_diff)))
+        return d
+
+    # median_filter_3x3 (stereo_matcher.hpp:91)
+    def median_filter_3x3(self, d):
+        v, keep = _image(d, np.float32)
+        out = _like(d, _shape(d), np.float32)
+        o, keep2 = _image(out, np.float32)
+        self._check(lib().r3dg_median_filter_3x3(self._ctx, v, o))
+        return out
+
+    # detail::ranges_from_coarse (stereo_matcher.hpp:121-125)
+    def ranges_from_coarse(self, coarse, width, height, fine_valid, partner_valid, partner_width, partner_sign, dmin,
+                           nd, options=None, mirrored=False):
+        o = (options or StereoOptions()).to_c()
+        cv, k1 = _image(coarse, np.float32)
+        fv, k2 = _image(fine_valid, np.uint8)
+        pv, k3 = _image(partner_valid, np.uint8)
+        lo = np.empty((height, width), np.uint16)
+        num = np.empty((height, width), np.uint16)
+        lov, k4 = _image(lo, np.uint16)
+        nv, k5 = _image(num, np.uint16)
+        self._check(lib().r3dg_ranges_from_coarse(self._ctx, cv, int(width), int(height), fv, pv, int(partner_width),
+                                                  int(partner_sign), int(dmin), int(nd), ctypes.byref(o),
+                                                  int(bool(mirrored)), lov, nv))
+        return lo, num
+
+
+def automatic_levels(w, h, nd):
+    """detail::automatic_levels (stereo_matcher.hpp:128)."""
+    return int(lib().r3dg_automatic_levels(int(w), int(h), int(nd)))
+
+
+def default_options() -> StereoOptions:
+    return StereoOptions()
