@@ -7,9 +7,9 @@
  * happen before a context exists go to r3dg_global_error().
  *
  * Images are strided 2-D views (r3dg_image).  `memory` says whether `data`
- * is host memory (the library copies through a pinned staging buffer) or
- * device memory of the context's GPU (used in place, no copy).  Outputs follow
- * the same rule.  All calls of one context run on that context's HIP stream
+ * is host memory (the library copies it to and from the device) or device
+ * memory of the context's GPU (used in place, no copy).  Outputs follow the
+ * same rule.  All calls of one context run on that context's HIP stream
  * and return after the results are complete, like the reference API they
  * replace; several host threads may share a context (calls are serialised by
  * a per-context mutex).
@@ -84,6 +84,19 @@ typedef struct r3dg_options {
   uint64_t memory_budget;
 } r3dg_options;
 
+/* Stages timed with HIP events when profiling is on (r3dg_set_profiling). */
+enum {
+  R3DG_STAGE_PYRAMID = 0,   /* pyramid_down of images and masks */
+  R3DG_STAGE_PREPARE = 1,   /* mirrors, Census, valid intervals */
+  R3DG_STAGE_RANGES = 2,    /* ranges_from_coarse */
+  R3DG_STAGE_LAYOUT = 3,    /* tile label layout (including its read-back) */
+  R3DG_STAGE_AGGREGATE = 4, /* the 8 SGM path kernels */
+  R3DG_STAGE_SELECT = 5,    /* WTA, sub-pixel, checks */
+  R3DG_STAGE_BLEND = 6,     /* tile blending */
+  R3DG_STAGE_FILTER = 7,    /* left-right check, speckle and median filters */
+  R3DG_STAGES = 8
+};
+
 /* Per-call statistics (DisparityMap fields, stereo_matcher.hpp:61-68, plus timings). */
 typedef struct r3dg_match_info {
   int32_t disparity_min;   /* DisparityMap::disparity_min */
@@ -93,6 +106,9 @@ typedef struct r3dg_match_info {
   uint64_t cells;          /* DisparityMap::cells: aggregated cells, all levels and directions */
   uint64_t tile_pixels;    /* sum of tile pixels over levels and directions */
   double seconds;          /* wall time of the call */
+  uint64_t aggregate_launches; /* path-kernel launches */
+  uint64_t aggregate_bytes;    /* algorithmic bytes of the path kernels: sum of 30 C + 176 P */
+  double stage_seconds[R3DG_STAGES]; /* device time per stage (0 unless profiling) */
 } r3dg_match_info;
 
 /* Optional integer outputs of r3dg_match at level 0 (views may be empty).
@@ -125,6 +141,10 @@ int32_t r3dg_ctx_destroy(r3dg_ctx* ctx);
  * and returns the status of that failure. */
 int32_t r3dg_last_error(const r3dg_ctx* ctx, char* buf, size_t size);
 int32_t r3dg_global_error(char* buf, size_t size);
+
+/* Records HIP events around every stage of later calls on the context's
+ * stream and reports their durations in r3dg_match_info::stage_seconds. */
+int32_t r3dg_set_profiling(r3dg_ctx* ctx, int32_t enabled);
 
 /* Cancellation flag polled between pyramid levels and tile stages; replaces
  * CancellationToken polling in parallel_for (core/thread_pool.cpp:41-70). */

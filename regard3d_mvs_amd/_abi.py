@@ -45,10 +45,15 @@ class COptions(ctypes.Structure):
                 ("tile_blend", ctypes.c_int32), ("memory_budget", ctypes.c_uint64)]
 
 
+STAGES = ("pyramid", "prepare", "ranges", "layout", "aggregate", "select", "blend", "filter")
+
+
 class CMatchInfo(ctypes.Structure):
     _fields_ = [("disparity_min", ctypes.c_int32), ("num_disparities", ctypes.c_int32),
                 ("levels", ctypes.c_int32), ("tiles_level0", ctypes.c_int32), ("cells", ctypes.c_uint64),
-                ("tile_pixels", ctypes.c_uint64), ("seconds", ctypes.c_double)]
+                ("tile_pixels", ctypes.c_uint64), ("seconds", ctypes.c_double),
+                ("aggregate_launches", ctypes.c_uint64), ("aggregate_bytes", ctypes.c_uint64),
+                ("stage_seconds", ctypes.c_double * len(STAGES))]
 
 
 class CMatchDebug(ctypes.Structure):
@@ -100,6 +105,9 @@ class DisparityMap:
     labels: object = None
     lr_mask: object = None
     seconds: float = 0.0
+    aggregate_launches: int = 0
+    aggregate_bytes: int = 0
+    stage_seconds: dict = dataclasses.field(default_factory=dict)
 
 
 def align16(n: int) -> int:

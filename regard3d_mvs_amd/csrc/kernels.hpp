@@ -51,11 +51,11 @@ void launch_ranges_from_coarse(const float* coarse, int wc, int hc, int width, i
                                std::uint16_t* lo, std::uint16_t* num, std::int64_t rs, hipStream_t s);
 
 // ---- aggregation (kernels_sgm.hip)
-// Cells of every tile row (row_cells[row_base + y]) and pixel offsets inside
-// the row (col_off); one block per tile row.
+// Cells of every tile row (row_cells[row_base + y]) and the pixel records
+// (cell offset inside the row, lo, num); one block per tile row.
 // row_max = largest label count of the row.
 void launch_tile_row_scan(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* row_prefix, int ntiles,
-                          std::uint64_t total_rows, std::uint32_t* col_off, std::uint64_t* row_cells,
+                          std::uint64_t total_rows, std::uint64_t* pix, std::uint64_t* row_cells,
                           std::uint32_t* row_max, hipStream_t s);
 // Exclusive scan of row_cells per tile into row_start; tile_cells[t] = total,
 // tile_max[t] = largest label count of the tile.
@@ -63,13 +63,18 @@ void launch_tile_cells(const TileDesc* tiles, int ntiles, const std::uint64_t* r
                        std::uint64_t* row_start, std::uint64_t* tile_cells, std::uint32_t* tile_max, hipStream_t s);
 // One SGM path over every tile (path index 0..7, see kernels_sgm.hip).
 void launch_sgm_path(int path, const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* group_prefix, int ntiles,
-                     std::uint64_t groups, const std::uint64_t* row_start, const std::uint32_t* col_off,
-                     std::uint16_t* S, bool accumulate, int max_num, hipStream_t s);
+                     std::uint64_t groups, const std::uint64_t* row_start, const std::uint64_t* pix,
+                     std::uint16_t* S, std::uint64_t total_cells, bool accumulate, int max_num, hipStream_t s);
+// Extra uint16 elements the volume S needs past its last cell (prefetch reads).
+constexpr std::size_t kSPad = 64;
 // WTA + sub-pixel + checks of every tile row into per-tile-pixel results.
 void launch_select(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* row_prefix, int ntiles,
-                   std::uint64_t total_rows, const std::uint64_t* row_start, const std::uint32_t* col_off,
+                   std::uint64_t total_rows, const std::uint64_t* row_start, const std::uint64_t* pix,
                    const std::uint16_t* S, SelectParams p, int max_w, int max_D, float* tdisp, float* tconf,
                    std::int16_t* tlab, hipStream_t s);
+// Packs two valid-interval arrays ([h][2] each) into rows4 ([h][4]).
+void launch_pack_rows(const std::int32_t* base, const std::int32_t* partner, int h, std::int32_t* rows4,
+                      hipStream_t s);
 // Blends the tiles of one job into a full-size map.
 void launch_blend(GridDesc g, const TileDesc* tiles, const float* tdisp, const float* tconf, const std::int16_t* tlab,
                   float* disp, float* conf, std::int16_t* lab, hipStream_t s);

@@ -176,13 +176,28 @@ __global__ void k_apply_mask(float* __restrict__ map, std::size_t n, const std::
 // remove_speckles (stereo_matcher.cpp:919-970) is a BFS whose segments are the
 // connected components of the 4-neighbour graph with edges
 // |d_j - d_i| <= max_diff (the relation is symmetric, so the segments do not
-// depend on the visiting order).  Here: lock-free union-find with atomicMin
-// (Playne & Hawick 2018), then component sizes, then invalidation.
+// depend on the visiting order).  Here: union-find in three steps, every
+// parent pointer only ever moving to a smaller pixel index (no cycles):
+//   1. each 32x8 block unions its pixels in LDS and points them to the
+//      block-local root;
+//   2. pixels on block borders union across them in HBM (atomicMin links,
+//      Playne & Hawick 2018, with path halving);
+//   3. every pixel finds its root and counts it, saturating at min_size (only
+//      "fewer than min_size" matters, and big segments would serialise on one
+//      counter), then small segments are invalidated.
+constexpr int kCcW = 32, kCcH = 8;
+
+__device__ __forceinline__ int ld_relaxed(int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ int cc_find(int* p, int i) {
-  int v = __hip_atomic_load(&p[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  while (v != i) {
-    i = v;
-    v = __hip_atomic_load(&p[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int par = ld_relaxed(&p[i]);
+  while (par != i) {
+    const int gp = ld_relaxed(&p[par]);
+    if (gp != par) atomicMin(&p[i], gp);  // path halving: skip to the grandparent
+    i = gp;
+    par = ld_relaxed(&p[i]);
   }
   return i;
 }
@@ -203,29 +218,70 @@ __device__ void cc_union(int* p, int a, int b) {
   }
 }
 
-__global__ void k_cc_init(int* __restrict__ p, int* __restrict__ size, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) {
-    p[i] = i;
+__device__ __forceinline__ int lds_find(int* lab, int i) {
+  int par = __hip_atomic_load(&lab[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  while (par != i) {
+    i = par;
+    par = __hip_atomic_load(&lab[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  return i;
+}
+
+__device__ void lds_union(int* lab, int a, int b) {
+  for (;;) {
+    a = lds_find(lab, a);
+    b = lds_find(lab, b);
+    if (a == b) return;
+    if (a > b) {
+      const int t = a;
+      a = b;
+      b = t;
+    }
+    const int old = atomicMin(&lab[b], a);
+    if (old == b) return;
+    b = old;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cc_local(const float* __restrict__ d, int w, int h, float max_diff,
+                                                  int* __restrict__ p, int* __restrict__ size) {
+  __shared__ int lab[kCcW * kCcH];
+  __shared__ float dv[kCcW * kCcH];
+  const int tx = threadIdx.x, ty = threadIdx.y, t = ty * kCcW + tx;
+  const int x = blockIdx.x * kCcW + tx, y = blockIdx.y * kCcH + ty;
+  const bool in = x < w && y < h;
+  dv[t] = in ? d[std::int64_t(y) * w + x] : __builtin_nanf("");
+  lab[t] = t;
+  __syncthreads();
+  if (in) {
+    const float v = dv[t];
+    if (tx + 1 < kCcW && x + 1 < w && fabsf(dv[t + 1] - v) <= max_diff) lds_union(lab, t, t + 1);
+    if (ty + 1 < kCcH && y + 1 < h && fabsf(dv[t + kCcW] - v) <= max_diff) lds_union(lab, t, t + kCcW);
+  }
+  __syncthreads();
+  if (in) {
+    const int r = lds_find(lab, t);
+    const std::int64_t i = std::int64_t(y) * w + x;
+    p[i] = (blockIdx.y * kCcH + r / kCcW) * w + blockIdx.x * kCcW + r % kCcW;
     size[i] = 0;
   }
 }
 
-__global__ void k_cc_union(const float* __restrict__ d, int w, int h, float max_diff, int* p) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  if (x >= w) return;
+__global__ __launch_bounds__(256) void k_cc_border(const float* __restrict__ d, int w, int h, float max_diff, int* p) {
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int x = blockIdx.x * kCcW + tx, y = blockIdx.y * kCcH + ty;
+  if (x >= w || y >= h) return;
   const int i = y * w + x;
-  const float v = d[i];
-  if (x + 1 < w && fabsf(d[i + 1] - v) <= max_diff) cc_union(p, i, i + 1);
-  if (y + 1 < h && fabsf(d[i + w] - v) <= max_diff) cc_union(p, i, i + w);
+  if (tx == kCcW - 1 && x + 1 < w && fabsf(d[i + 1] - d[i]) <= max_diff) cc_union(p, i, i + 1);
+  if (ty == kCcH - 1 && y + 1 < h && fabsf(d[i + w] - d[i]) <= max_diff) cc_union(p, i, i + w);
 }
 
-__global__ void k_cc_count(int* p, int* __restrict__ size, int n) {
+__global__ void k_cc_count(int* p, int* __restrict__ size, int n, int min_size) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int r = cc_find(p, i);
-  p[i] = r;
-  atomicAdd(&size[r], 1);
+  __hip_atomic_store(&p[i], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (ld_relaxed(&size[r]) < min_size) atomicAdd(&size[r], 1);
 }
 
 __global__ void k_cc_apply(float* __restrict__ d, const int* __restrict__ p, const int* __restrict__ size, int n,
@@ -474,9 +530,10 @@ void launch_speckles(float* d, int w, int h, int min_size, float max_diff, std::
                      hipStream_t s) {
   const int n = w * h;
   if (min_size <= 1 || n == 0) return;
-  k_cc_init<<<(n + 255) / 256, 256, 0, s>>>(parent, size, n);
-  k_cc_union<<<dim3((w + 255) / 256, h), 256, 0, s>>>(d, w, h, max_diff, parent);
-  k_cc_count<<<(n + 255) / 256, 256, 0, s>>>(parent, size, n);
+  const dim3 b(kCcW, kCcH), g((w + kCcW - 1) / kCcW, (h + kCcH - 1) / kCcH);
+  k_cc_local<<<g, b, 0, s>>>(d, w, h, max_diff, parent, size);
+  k_cc_border<<<g, b, 0, s>>>(d, w, h, max_diff, parent);
+  k_cc_count<<<(n + 255) / 256, 256, 0, s>>>(parent, size, n, min_size);
   k_cc_apply<<<(n + 255) / 256, 256, 0, s>>>(d, parent, size, n, min_size);
 }
 

@@ -40,6 +40,18 @@ __device__ __forceinline__ int find_tile(const std::uint64_t* __restrict__ prefi
   return lo;
 }
 
+// Pointers read out of JobDesc / TileDesc are generic to the compiler; these
+// views make their loads and stores global_* instructions (flat ones also count
+// against the LDS wait counter, which would serialise the prefetch below).
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* gptr(const T* p) {
+  return (const __attribute__((address_space(1))) T*)(p);
+}
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* gptr_mut(T* p) {
+  return (__attribute__((address_space(1))) T*)(p);
+}
+
 __device__ __forceinline__ void label_range(const JobDesc& J, int X, int Y, int& lo, int& num) {
   if (J.lo != nullptr) {
     lo = J.lo[Y * J.r_stride + X];
@@ -52,11 +64,11 @@ __device__ __forceinline__ void label_range(const JobDesc& J, int X, int Y, int&
 
 // ------------------------------------------------------------------ layout
 // Exclusive scan of the label counts of one tile row (block-wide, in chunks
-// of 256 pixels): col_off = cell offset of each pixel inside its row.
+// of 256 pixels) into the pixel records: cell offset inside the row, lo, num.
 __global__ __launch_bounds__(256) void k_tile_row_scan(const JobDesc* __restrict__ jobs,
                                                        const TileDesc* __restrict__ tiles,
                                                        const std::uint64_t* __restrict__ row_prefix, int ntiles,
-                                                       std::uint32_t* __restrict__ col_off,
+                                                       std::uint64_t* __restrict__ pix,
                                                        std::uint64_t* __restrict__ row_cells,
                                                        std::uint32_t* __restrict__ row_max) {
   const std::uint64_t r = blockIdx.x;
@@ -73,8 +85,8 @@ __global__ __launch_bounds__(256) void k_tile_row_scan(const JobDesc* __restrict
   for (int base = 0; base < T.w; base += 256) {
     const int x = base + threadIdx.x;
     std::uint32_t v = 0;
+    int lo = 0, num = 0;
     if (x < T.w) {
-      int lo, num;
       label_range(J, T.x0 + x, Y, lo, num);
       v = static_cast<std::uint32_t>(num);
       atomicMax(&smax, v);
@@ -87,7 +99,7 @@ __global__ __launch_bounds__(256) void k_tile_row_scan(const JobDesc* __restrict
       sc[threadIdx.x] += add;
       __syncthreads();
     }
-    if (x < T.w) col_off[T.px_base + std::uint64_t(y) * T.w + x] = carry + sc[threadIdx.x] - v;
+    if (x < T.w) pix[T.px_base + std::uint64_t(y) * T.w + x] = pix_record(carry + sc[threadIdx.x] - v, lo, num);
     carry += sc[255];
     __syncthreads();
   }
@@ -166,91 +178,186 @@ __device__ __forceinline__ void scan_start(int s, int w, int h, int& x, int& y, 
   }
 }
 
-__device__ __forceinline__ std::uint32_t min16(std::uint32_t v) {
-  v = min(v, static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(v), 8, 16)));
-  v = min(v, static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(v), 4, 16)));
-  v = min(v, static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(v), 2, 16)));
-  v = min(v, static_cast<std::uint32_t>(__shfl_xor(static_cast<int>(v), 1, 16)));
+// Min over the 16 lanes of a DPP row, broadcast to all of them (row_ror).
+__device__ __forceinline__ std::uint32_t row_min16(std::uint32_t v) {
+  v = min(v, static_cast<std::uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x128, 0xF, 0xF, false)));
+  v = min(v, static_cast<std::uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x124, 0xF, 0xF, false)));
+  v = min(v, static_cast<std::uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x122, 0xF, 0xF, false)));
+  v = min(v, static_cast<std::uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x121, 0xF, 0xF, false)));
   return v;
 }
 
-template <int DX, int DY>
-__global__ __launch_bounds__(256) void k_path(const JobDesc* __restrict__ jobs, const TileDesc* __restrict__ tiles,
+// Per-step values of one scanline pixel that do not depend on the recursion.
+struct StepIn {
+  std::uint64_t rec;  // pixel record
+  std::uint64_t rs;   // row start of the pixel's tile row
+  std::uint64_t cl;   // base Census
+  int4 rows;          // valid columns of the row (base first/last, partner first/last)
+  int ip;             // base intensity
+};
+
+// Labels j = lane + 16 m of the first kPre chunks, loaded one step ahead.
+template <int kPre>
+struct StepLanes {
+  std::uint32_t s[kPre];   // current S (accumulating paths)
+  std::uint64_t cr[kPre];  // partner Census at column X - dmin - lo - j (clamped into the image)
+};
+
+// One path direction over every tile.  The loop over the scanline's pixels is
+// software-pipelined: the pixel records, Census and row data of step i + 2
+// and the S / partner-Census values of step i + 1 are in flight while step i
+// computes, with no branch around them, so the compiler's wait counters stay
+// exact.  Steps past the end of a scanline (scanlines of a wave differ in
+// length) and label chunks past a pixel's range run with clamped loads and
+// send their stores to a per-lane dummy slot past the end of S.
+template <int DX, int DY, int kPre, bool kAcc>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_path(const JobDesc* __restrict__ jobs, const TileDesc* __restrict__ tiles,
                                               const std::uint64_t* __restrict__ gprefix, int ntiles,
                                               std::uint64_t groups, const std::uint64_t* __restrict__ row_start,
-                                              const std::uint32_t* __restrict__ col_off, std::uint16_t* __restrict__ S,
-                                              int accumulate, int max_num) {
+                                              const std::uint64_t* __restrict__ pix, std::uint16_t* __restrict__ S,
+                                              std::uint64_t total_cells, int max_num) {
   extern __shared__ std::uint16_t lds[];
   const int waves = blockDim.x >> 6;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63, grp = lane >> 4, l16 = lane & 15;
   const std::uint64_t gid = std::uint64_t(blockIdx.x) * waves + wave;
-  if (gid >= groups) return;
-  const int t = find_tile(gprefix, ntiles, gid);
+  const bool wave_active = gid < groups;
+  const int t = find_tile(gprefix, ntiles, wave_active ? gid : gprefix[ntiles] - 1);
   const TileDesc T = tiles[t];
   const JobDesc& J = jobs[T.job];
-  const int s = static_cast<int>(gid - gprefix[t]) * 4 + grp;
-  if (s >= scanlines<DX, DY>(T.w, T.h)) return;
+  std::uint16_t* my_lut = lds + wave * 256;  // P2 table of the wave's job
+  for (int k = lane; k < 256; k += 64) my_lut[k] = J.lut[k];
+  __syncthreads();
+  if (!wave_active) return;
+  const int ns = scanlines<DX, DY>(T.w, T.h);
+  const int s0 = static_cast<int>(gid - gprefix[t]) * 4;
+  const int s = min(s0 + grp, ns - 1);     // idle groups shadow the last scanline
+  const bool grp_active = s0 + grp < ns;
+  int x0, y0, len;
+  scan_start<DX, DY>(s, T.w, T.h, x0, y0, len);
+  // Steps of the wave: the longest of its scanlines.
+  int steps = grp_active ? len : 0;
+  steps = max(steps, __shfl_xor(steps, 16));
+  steps = max(steps, __shfl_xor(steps, 32));
+  steps = __builtin_amdgcn_readfirstlane(steps);
 
-  std::uint16_t* buf = lds + std::size_t(wave * 4 + grp) * 2 * max_num;
-  int x, y, len;
-  scan_start<DX, DY>(s, T.w, T.h, x, y, len);
+  // LDS row of a scanline: the labels plus room for the masked-off chunks of a
+  // 4-chunk batch (their writes land past the row's labels and are never read).
+  const int rowlen = (max_num > 16 * kPre ? max_num : 16 * kPre) + 48;
+  std::uint16_t* buf = lds + waves * 256 + std::size_t(wave * 4 + grp) * 2 * rowlen;
   const std::uint32_t P1 = static_cast<std::uint32_t>(J.p1);
   const std::uint32_t invalid = static_cast<std::uint32_t>(J.invalid_cost);
-  int prev_lo = 0, prev_num = 0;
-  std::uint32_t prev_min = 0;
-  for (int i = 0; i < len; ++i, x += DX, y += DY) {
+  const int pw1 = J.pwidth - 1;
+  const auto tpix = gptr(pix + T.px_base);
+  const auto trs = gptr(row_start + T.row_base);
+  const auto gcl = gptr(J.cl);
+  const auto gcr = gptr(J.cr);
+  const auto gimg = gptr(J.img);
+  const auto grows = gptr(J.rows4);
+  const auto gS = gptr_mut(S);
+  const auto dummy = gS + total_cells + lane;  // inside the padding of S
+
+  auto load_in = [&](int i, StepIn& u) {
+    i = min(i, len - 1);
+    const int x = x0 + i * DX, y = y0 + i * DY;
     const int X = T.x0 + x, Y = T.y0 + y;
-    int lo, num;
-    label_range(J, X, Y, lo, num);
-    const std::uint64_t cell = T.cell_base + row_start[T.row_base + y] + col_off[T.px_base + std::uint64_t(y) * T.w + x];
-    const std::uint64_t cl = J.cl[Y * J.cl_stride + X];
-    const int lf = J.lrows[2 * Y], ll = J.lrows[2 * Y + 1];
-    const int rf = J.rrows[2 * Y], rl = J.rrows[2 * Y + 1];
-    const bool lvalid = X >= lf && X <= ll;
-    std::uint32_t pen = 0;
-    if (i > 0) {
-      const int ip = J.img[Y * J.img_stride + X];
-      const int iq = J.img[(Y - DY) * J.img_stride + (X - DX)];
-      pen = min(prev_min + J.lut[abs(ip - iq) >> 8], 0xFFFFU);
+    u.rec = tpix[std::int64_t(y) * T.w + x];
+    u.rs = trs[y];
+    u.cl = gcl[Y * J.cl_stride + X];
+    u.rows = make_int4(grows[4 * Y], grows[4 * Y + 1], grows[4 * Y + 2], grows[4 * Y + 3]);
+    u.ip = gimg[Y * J.img_stride + X];
+  };
+  auto load_lanes = [&](int i, const StepIn& u, StepLanes<kPre>& v) {
+    i = min(i, len - 1);
+    const int X = T.x0 + x0 + i * DX, Y = T.y0 + y0 + i * DY;
+    const std::uint64_t cell = T.cell_base + u.rs + pix_off(u.rec);
+    const int xr0 = X - J.dmin - pix_lo(u.rec) - l16;
+    const auto crow = gcr + Y * J.cr_stride;
+#pragma unroll
+    for (int m = 0; m < kPre; ++m) {
+      // Chunks past the pixel's labels read neighbouring (padded) cells and a
+      // clamped column; their values are never used.
+      if constexpr (kAcc) v.s[m] = gS[cell + l16 + 16 * m];
+      v.cr[m] = crow[min(max(xr0 - 16 * m, 0), pw1)];
     }
-    std::uint16_t* cur = buf + (i & 1) * max_num;
-    const std::uint16_t* prv = buf + ((i + 1) & 1) * max_num;
-    const std::uint64_t* crow = J.cr + Y * J.cr_stride;
+  };
+
+  int prev_lo = 0, prev_num = 0, prev_ip = 0;
+  std::uint32_t prev_min = 0;
+  int flip = 0;
+  // Step i with its inputs u (this step), un (next) and un2 (receives step
+  // i + 2), v (this step) and vn (receives step i + 1).
+  auto step = [&](int i, StepIn& u, StepIn& un, StepIn& un2, StepLanes<kPre>& v, StepLanes<kPre>& vn) {
+    load_in(i + 2, un2);
+    load_lanes(i + 1, un, vn);
+    const bool act = grp_active && i < len;
+    const int ic = min(i, len - 1);
+    const int X = T.x0 + x0 + ic * DX;
+    const int lo = pix_lo(u.rec), num = pix_num(u.rec);
+    const std::uint64_t cell = T.cell_base + u.rs + pix_off(u.rec);
+    const bool lvalid = X >= u.rows.x && X <= u.rows.y;
+    const int rf = u.rows.z, rl = u.rows.w;
+    const bool first = i == 0;
+    const std::uint32_t pen = min(prev_min + my_lut[abs(u.ip - prev_ip) >> 8], 0xFFFFU);
+    std::uint16_t* cur = buf + flip * rowlen;
+    const std::uint16_t* prv = buf + (flip ^ 1) * rowlen;
     const int xr0 = X - J.dmin - lo;
-    std::uint16_t* srow = S + cell;
+    const int delta = lo - prev_lo;
     std::uint32_t m = 0xFFFFU;
-    for (int j = l16; j < num; j += 16) {
-      std::uint32_t c = 0;
-      if (lvalid) {
-        const int xr = xr0 - j;
-        c = xr >= rf && xr <= rl ? static_cast<std::uint32_t>(__popcll(cl ^ crow[xr])) : invalid;
-      }
-      std::uint32_t L = c;
-      if (i > 0) {
-        const int k = lo + j - prev_lo;
-        const std::uint32_t same = static_cast<unsigned>(k) < static_cast<unsigned>(prev_num) ? prv[k] : 0xFFFFU;
-        const std::uint32_t a = static_cast<unsigned>(k - 1) < static_cast<unsigned>(prev_num) ? prv[k - 1] : 0xFFFFU;
-        const std::uint32_t b = static_cast<unsigned>(k + 1) < static_cast<unsigned>(prev_num) ? prv[k + 1] : 0xFFFFU;
-        const std::uint32_t nb = min(min(a, b) + P1, 0xFFFFU);
-        const std::uint32_t best = min(min(same, nb), pen);
-        L = (best - prev_min + c) & 0xFFFFU;
-      }
-      cur[j] = static_cast<std::uint16_t>(L);
-      m = min(m, L);
-      if (accumulate) {
-        srow[j] = static_cast<std::uint16_t>(srow[j] + L);
-      } else {
-        srow[j] = static_cast<std::uint16_t>(L);
+    auto label = [&](int j, bool on, std::uint32_t sv, std::uint64_t crv) {
+      const int xr = xr0 - j;
+      std::uint32_t c = xr >= rf && xr <= rl ? static_cast<std::uint32_t>(__popcll(u.cl ^ crv)) : invalid;
+      c = lvalid ? c : 0U;
+      const int k = j + delta;
+      const std::uint32_t same = static_cast<unsigned>(k) < static_cast<unsigned>(prev_num) ? prv[k] : 0xFFFFU;
+      const std::uint32_t a = static_cast<unsigned>(k - 1) < static_cast<unsigned>(prev_num) ? prv[k - 1] : 0xFFFFU;
+      const std::uint32_t b = static_cast<unsigned>(k + 1) < static_cast<unsigned>(prev_num) ? prv[k + 1] : 0xFFFFU;
+      const std::uint32_t nb = min(min(a, b) + P1, 0xFFFFU);
+      const std::uint32_t best = min(min(same, nb), pen);
+      const std::uint32_t L = first ? c : ((best - prev_min + c) & 0xFFFFU);
+      cur[j] = static_cast<std::uint16_t>(L);  // j < rowlen; unused past num
+      m = on ? min(m, L) : m;
+      const auto dst = on ? gS + cell + j : dummy;
+      *dst = static_cast<std::uint16_t>(kAcc ? sv + L : L);
+    };
+    const int chunks = num >> 4;
+#pragma unroll
+    for (int q = 0; q < kPre; ++q) label(l16 + 16 * q, act && q < chunks, v.s[q], v.cr[q]);
+    if (chunks > kPre) {  // wide ranges: batches of 4 chunks, loads issued together
+      const auto crow = gcr + (T.y0 + y0 + ic * DY) * J.cr_stride;
+      for (int q = kPre; q < chunks; q += 4) {
+        std::uint32_t sv[4];
+        std::uint64_t cv[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int j = min(l16 + 16 * (q + b), num - 1);
+          sv[b] = kAcc ? static_cast<std::uint32_t>(gS[cell + j]) : 0U;
+          cv[b] = crow[min(max(xr0 - j, 0), pw1)];
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) label(l16 + 16 * (q + b), act && q + b < chunks, sv[b], cv[b]);
       }
     }
-    prev_min = min16(m);
+    prev_min = row_min16(m);
     prev_lo = lo;
     prev_num = num;
-    // The next step reads this step's LDS row (other lanes' values).
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    prev_ip = u.ip;
+    flip ^= 1;
+    __builtin_amdgcn_wave_barrier();  // keeps this step's LDS writes before the next step's reads
+  };
+
+  StepIn ua, ub, uc;
+  StepLanes<kPre> va, vb;
+  load_in(0, ua);
+  load_in(1, ub);
+  load_lanes(0, ua, va);
+  for (int i = 0; i < steps; i += 6) {
+    step(i + 0, ua, ub, uc, va, vb);
+    step(i + 1, ub, uc, ua, vb, va);
+    step(i + 2, uc, ua, ub, va, vb);
+    step(i + 3, ua, ub, uc, vb, va);
+    step(i + 4, ub, uc, ua, va, vb);
+    step(i + 5, uc, ua, ub, vb, va);
   }
 }
 
@@ -303,7 +410,7 @@ __device__ __forceinline__ void wta(const std::uint16_t* __restrict__ s, int n, 
 __global__ __launch_bounds__(256) void k_select(const JobDesc* __restrict__ jobs, const TileDesc* __restrict__ tiles,
                                                 const std::uint64_t* __restrict__ row_prefix, int ntiles,
                                                 const std::uint64_t* __restrict__ row_start,
-                                                const std::uint32_t* __restrict__ col_off,
+                                                const std::uint64_t* __restrict__ pix,
                                                 const std::uint16_t* __restrict__ S, SelectParams p,
                                                 float* __restrict__ tdisp, float* __restrict__ tconf,
                                                 std::int16_t* __restrict__ tlab) {
@@ -314,8 +421,8 @@ __global__ __launch_bounds__(256) void k_select(const JobDesc* __restrict__ jobs
   const JobDesc& J = jobs[T.job];
   const int y = static_cast<int>(r - row_prefix[t]);
   const int Y = T.y0 + y;
-  const int lf = J.lrows[2 * Y], ll = J.lrows[2 * Y + 1];
-  const int rf = J.rrows[2 * Y], rl = J.rrows[2 * Y + 1];
+  const int4 rows = reinterpret_cast<const int4*>(J.rows4)[Y];
+  const int lf = rows.x, ll = rows.y, rf = rows.z, rl = rows.w;
   const int xb = min(max(lf - T.x0, 0), T.w);
   const int xe = min(max(ll + 1 - T.x0, xb), T.w);
   const std::uint64_t rowcell = T.cell_base + row_start[T.row_base + y];
@@ -328,9 +435,9 @@ __global__ __launch_bounds__(256) void k_select(const JobDesc* __restrict__ jobs
     for (int i = threadIdx.x; i < T.w + D; i += blockDim.x) keys[i] = 0xFFFFFFFFU;
     __syncthreads();
     for (int x = xb + threadIdx.x; x < xe; x += blockDim.x) {
-      int lo, num;
-      label_range(J, T.x0 + x, Y, lo, num);
-      const std::uint16_t* s = S + rowcell + col_off[px + x];
+      const std::uint64_t rec = pix[px + x];
+      const int lo = pix_lo(rec), num = pix_num(rec);
+      const std::uint16_t* s = S + rowcell + pix_off(rec);
       for (int k = 0; k < num; ++k) {
         atomicMin(&keys[x - (lo + k) + D - 1], (std::uint32_t(s[k]) << 16) | std::uint32_t(x));
       }
@@ -343,9 +450,9 @@ __global__ __launch_bounds__(256) void k_select(const JobDesc* __restrict__ jobs
     std::int16_t lab = INT16_MIN;
     if (x >= xb && x < xe) {
       const int X = T.x0 + x;
-      int lo, n;
-      label_range(J, X, Y, lo, n);
-      const std::uint16_t* s = S + rowcell + col_off[px + x];
+      const std::uint64_t rec = pix[px + x];
+      const int lo = pix_lo(rec), n = pix_num(rec);
+      const std::uint16_t* s = S + rowcell + pix_off(rec);
       int kb;
       std::uint32_t bc, second;
       wta(s, n, kb, bc, second);
@@ -460,24 +567,40 @@ __global__ void k_blend(GridDesc g, const TileDesc* __restrict__ tiles, const fl
   if (lab != nullptr) lab[o] = lv;
 }
 
+__global__ void k_pack_rows(const std::int32_t* __restrict__ base, const std::int32_t* __restrict__ partner, int h,
+                            std::int32_t* __restrict__ rows4) {
+  const int y = blockIdx.x * blockDim.x + threadIdx.x;
+  if (y >= h) return;
+  reinterpret_cast<int4*>(rows4)[y] = make_int4(base[2 * y], base[2 * y + 1], partner[2 * y], partner[2 * y + 1]);
+}
+
 template <int DX, int DY>
 void path_launch(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* gp, int ntiles,
-                 std::uint64_t groups, const std::uint64_t* rs, const std::uint32_t* co, std::uint16_t* S,
-                 bool accumulate, int max_num, hipStream_t s) {
-  // Four scanlines per wave; 2 LDS rows of max_num labels per scanline.
-  const int waves = max_num <= 1024 ? 4 : 1;
-  const std::size_t lds = std::size_t(waves) * 4 * 2 * max_num * sizeof(std::uint16_t);
+                 std::uint64_t groups, const std::uint64_t* rs, const std::uint64_t* pix, std::uint16_t* S,
+                 std::uint64_t total_cells, bool accumulate, int max_num, hipStream_t s) {
+  // Four scanlines per wave; 2 LDS rows of max_num labels per scanline plus
+  // one 256-entry P2 table per wave.
+  constexpr int kPre = 2;
+  const int rowlen = std::max(max_num, 16 * kPre) + 48;  // as in k_path
+  const auto lds_of = [&](int w) { return std::size_t(w) * (256 + 4 * 2 * std::size_t(rowlen)) * sizeof(std::uint16_t); };
+  const int waves = lds_of(4) <= 65536 ? 4 : 1;
+  const std::size_t lds = lds_of(waves);
   const std::uint64_t blocks = (groups + waves - 1) / waves;
-  k_path<DX, DY><<<dim3(static_cast<unsigned>(blocks)), 64 * waves, lds, s>>>(jobs, tiles, gp, ntiles, groups, rs, co,
-                                                                              S, accumulate ? 1 : 0, max_num);
+  if (accumulate) {
+    k_path<DX, DY, kPre, true><<<dim3(static_cast<unsigned>(blocks)), 64 * waves, lds, s>>>(
+        jobs, tiles, gp, ntiles, groups, rs, pix, S, total_cells, max_num);
+  } else {
+    k_path<DX, DY, kPre, false><<<dim3(static_cast<unsigned>(blocks)), 64 * waves, lds, s>>>(
+        jobs, tiles, gp, ntiles, groups, rs, pix, S, total_cells, max_num);
+  }
 }
 
 }  // namespace
 
 void launch_tile_row_scan(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* row_prefix, int ntiles,
-                          std::uint64_t total_rows, std::uint32_t* col_off, std::uint64_t* row_cells,
+                          std::uint64_t total_rows, std::uint64_t* pix, std::uint64_t* row_cells,
                           std::uint32_t* row_max, hipStream_t s) {
-  k_tile_row_scan<<<dim3(static_cast<unsigned>(total_rows)), 256, 0, s>>>(jobs, tiles, row_prefix, ntiles, col_off,
+  k_tile_row_scan<<<dim3(static_cast<unsigned>(total_rows)), 256, 0, s>>>(jobs, tiles, row_prefix, ntiles, pix,
                                                                           row_cells, row_max);
 }
 
@@ -487,28 +610,33 @@ void launch_tile_cells(const TileDesc* tiles, int ntiles, const std::uint64_t* r
 }
 
 void launch_sgm_path(int path, const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* gp, int ntiles,
-                     std::uint64_t groups, const std::uint64_t* rs, const std::uint32_t* co, std::uint16_t* S,
-                     bool accumulate, int max_num, hipStream_t s) {
+                     std::uint64_t groups, const std::uint64_t* rs, const std::uint64_t* co, std::uint16_t* S,
+                     std::uint64_t total_cells, bool accumulate, int max_num, hipStream_t s) {
   if (groups == 0) return;
   switch (path) {
-    case 0: path_launch<1, 0>(jobs, tiles, gp, ntiles, groups, rs, co, S, accumulate, max_num, s); break;
-    case 1: path_launch<-1, 0>(jobs, tiles, gp, ntiles, groups, rs, co, S, accumulate, max_num, s); break;
-    case 2: path_launch<0, 1>(jobs, tiles, gp, ntiles, groups, rs, co, S, accumulate, max_num, s); break;
-    case 3: path_launch<0, -1>(jobs, tiles, gp, ntiles, groups, rs, co, S, accumulate, max_num, s); break;
-    case 4: path_launch<1, 1>(jobs, tiles, gp, ntiles, groups, rs, co, S, accumulate, max_num, s); break;
-    case 5: path_launch<-1, -1>(jobs, tiles, gp, ntiles, groups, rs, co, S, accumulate, max_num, s); break;
-    case 6: path_launch<-1, 1>(jobs, tiles, gp, ntiles, groups, rs, co, S, accumulate, max_num, s); break;
-    default: path_launch<1, -1>(jobs, tiles, gp, ntiles, groups, rs, co, S, accumulate, max_num, s); break;
+    case 0: path_launch<1, 0>(jobs, tiles, gp, ntiles, groups, rs, co, S, total_cells, accumulate, max_num, s); break;
+    case 1: path_launch<-1, 0>(jobs, tiles, gp, ntiles, groups, rs, co, S, total_cells, accumulate, max_num, s); break;
+    case 2: path_launch<0, 1>(jobs, tiles, gp, ntiles, groups, rs, co, S, total_cells, accumulate, max_num, s); break;
+    case 3: path_launch<0, -1>(jobs, tiles, gp, ntiles, groups, rs, co, S, total_cells, accumulate, max_num, s); break;
+    case 4: path_launch<1, 1>(jobs, tiles, gp, ntiles, groups, rs, co, S, total_cells, accumulate, max_num, s); break;
+    case 5: path_launch<-1, -1>(jobs, tiles, gp, ntiles, groups, rs, co, S, total_cells, accumulate, max_num, s); break;
+    case 6: path_launch<-1, 1>(jobs, tiles, gp, ntiles, groups, rs, co, S, total_cells, accumulate, max_num, s); break;
+    default: path_launch<1, -1>(jobs, tiles, gp, ntiles, groups, rs, co, S, total_cells, accumulate, max_num, s); break;
   }
 }
 
 void launch_select(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* row_prefix, int ntiles,
-                   std::uint64_t total_rows, const std::uint64_t* row_start, const std::uint32_t* col_off,
+                   std::uint64_t total_rows, const std::uint64_t* row_start, const std::uint64_t* col_off,
                    const std::uint16_t* S, SelectParams p, int max_w, int max_D, float* tdisp, float* tconf,
                    std::int16_t* tlab, hipStream_t s) {
   const std::size_t lds = p.diagonal_lr ? std::size_t(max_w + max_D) * sizeof(std::uint32_t) : 0;
   k_select<<<dim3(static_cast<unsigned>(total_rows)), 256, lds, s>>>(jobs, tiles, row_prefix, ntiles, row_start,
                                                                       col_off, S, p, tdisp, tconf, tlab);
+}
+
+void launch_pack_rows(const std::int32_t* base, const std::int32_t* partner, int h, std::int32_t* rows4,
+                      hipStream_t s) {
+  k_pack_rows<<<dim3((h + 255) / 256), 256, 0, s>>>(base, partner, h, rows4);
 }
 
 void launch_blend(GridDesc g, const TileDesc* tiles, const float* tdisp, const float* tconf, const std::int16_t* tlab,

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <optional>
 #include <string>
 #include <vector>
 
@@ -239,12 +240,20 @@ struct Job {
   DImg<std::uint64_t> cl, cr;
   const std::int32_t* lrows = nullptr;
   const std::int32_t* rrows = nullptr;
+  DBuf<std::int32_t> rows4;
   const Ranges* ranges = nullptr;
   int dmin = 0;
   int D = 0;
   bool confidence = true;
   bool labels = false;
 };
+
+DBuf<std::int32_t> pack_rows(Context& c, const std::int32_t* base, const std::int32_t* partner, int h) {
+  DBuf<std::int32_t> r(c.arena, 4 * std::size_t(h));
+  launch_pack_rows(base, partner, h, r.get(), c.stream);
+  check_launch();
+  return r;
+}
 
 struct TilesOut {
   DImg<float> disp, conf;
@@ -290,7 +299,23 @@ int path_scanlines(int p, int w, int h) { return p < 2 ? h : (p < 4 ? w : w + h 
 
 // match_tiles (stereo_matcher.cpp:613-771) for several jobs at once: their
 // tiles share the layout pass, the 8 path launches and the selection launch.
-std::vector<TilesOut> match_tiles(Context& c, std::vector<Job>& jobs, const r3dg_options& o, SelectParams sp) {
+// Tile layout and aggregated volume S of a set of jobs (the tiles of all jobs
+// share the layout pass and the 8 path launches).
+struct Aggregation {
+  DBuf<JobDesc> djobs;
+  std::vector<TileDesc> td;
+  DBuf<TileDesc> dtiles;
+  std::vector<Grid> grids;
+  std::vector<int> tile_base;
+  std::vector<std::uint64_t> tcells;
+  DBuf<std::uint64_t> drow_prefix, row_start, pix;
+  DBuf<std::uint16_t> S;
+  std::uint64_t total_rows = 0, total_px = 0, total_cells = 0;
+  int max_w = 1, max_D = 16;
+};
+
+Aggregation aggregate_jobs(Context& c, std::vector<Job>& jobs, const r3dg_options& o) {
+  Aggregation A;
   const int nj = static_cast<int>(jobs.size());
   const auto lut = p2_table(o);
   std::vector<JobDesc> jd(static_cast<std::size_t>(nj));
@@ -304,8 +329,7 @@ std::vector<TilesOut> match_tiles(Context& c, std::vector<Job>& jobs, const r3dg
     d.cl_stride = J.cl.s;
     d.cr = J.cr.p;
     d.cr_stride = J.cr.s;
-    d.lrows = J.lrows;
-    d.rrows = J.rrows;
+    d.rows4 = J.rows4.get();
     d.lo = J.ranges != nullptr ? J.ranges->lo.p : nullptr;
     d.num = J.ranges != nullptr ? J.ranges->num.p : nullptr;
     d.r_stride = J.ranges != nullptr ? J.ranges->lo.s : 0;
@@ -319,21 +343,29 @@ std::vector<TilesOut> match_tiles(Context& c, std::vector<Job>& jobs, const r3dg
     d.confidence = J.confidence ? 1 : 0;
     std::memcpy(d.lut, lut.data(), sizeof(d.lut));
   }
-  DBuf<JobDesc> djobs;
+  DBuf<JobDesc>& djobs = A.djobs;
   upload(c, djobs, jd);
 
   const int ext = o.tile_margin + (o.tile_blend + 1) / 2;
   std::vector<int> tsize(static_cast<std::size_t>(nj), o.tile_size);
-  std::vector<Grid> grids(static_cast<std::size_t>(nj));
-  std::vector<TileDesc> td;
-  std::vector<int> tile_base(std::size_t(nj) + 1);
-  std::vector<std::uint64_t> row_prefix, tcells;
+  std::vector<Grid>& grids = A.grids;
+  grids.resize(static_cast<std::size_t>(nj));
+  std::vector<TileDesc>& td = A.td;
+  std::vector<int>& tile_base = A.tile_base;
+  tile_base.resize(std::size_t(nj) + 1);
+  std::vector<std::uint64_t> row_prefix;
+  std::vector<std::uint64_t>& tcells = A.tcells;
   std::vector<std::uint32_t> tmax;
-  DBuf<TileDesc> dtiles;
-  DBuf<std::uint64_t> drow_prefix, row_cells, row_start, tile_cells;
-  DBuf<std::uint32_t> col_off, row_max, tile_max;
-  std::uint64_t total_rows = 0, total_px = 0;
+  DBuf<TileDesc>& dtiles = A.dtiles;
+  DBuf<std::uint64_t>& drow_prefix = A.drow_prefix;
+  DBuf<std::uint64_t>& row_start = A.row_start;
+  DBuf<std::uint64_t>& pix = A.pix;
+  DBuf<std::uint64_t> row_cells, tile_cells;
+  DBuf<std::uint32_t> row_max, tile_max;
+  std::uint64_t& total_rows = A.total_rows;
+  std::uint64_t& total_px = A.total_px;
   std::vector<bool> fixed(static_cast<std::size_t>(nj), false);
+  std::optional<StageTimer> layout_timer(std::in_place, c, R3DG_STAGE_LAYOUT);
   for (;;) {
     td.clear();
     row_prefix.clear();
@@ -364,13 +396,13 @@ std::vector<TilesOut> match_tiles(Context& c, std::vector<Job>& jobs, const r3dg
     const int nt = static_cast<int>(td.size());
     upload(c, dtiles, td);
     upload(c, drow_prefix, row_prefix);
-    col_off = DBuf<std::uint32_t>(c.arena, total_px);
+    pix = DBuf<std::uint64_t>(c.arena, total_px);
     row_cells = DBuf<std::uint64_t>(c.arena, total_rows);
     row_max = DBuf<std::uint32_t>(c.arena, total_rows);
     row_start = DBuf<std::uint64_t>(c.arena, total_rows);
     tile_cells = DBuf<std::uint64_t>(c.arena, std::size_t(nt));
     tile_max = DBuf<std::uint32_t>(c.arena, std::size_t(nt));
-    launch_tile_row_scan(djobs.get(), dtiles.get(), drow_prefix.get(), nt, total_rows, col_off.get(), row_cells.get(),
+    launch_tile_row_scan(djobs.get(), dtiles.get(), drow_prefix.get(), nt, total_rows, pix.get(), row_cells.get(),
                          row_max.get(), c.stream);
     check_launch();
     launch_tile_cells(dtiles.get(), nt, row_cells.get(), row_max.get(), row_start.get(), tile_cells.get(),
@@ -403,8 +435,10 @@ std::vector<TilesOut> match_tiles(Context& c, std::vector<Job>& jobs, const r3dg
     if (!changed) break;
   }
   const int nt = static_cast<int>(td.size());
-  std::uint64_t total_cells = 0;
-  int max_num = 16, max_w = 1, max_D = 16;
+  std::uint64_t& total_cells = A.total_cells;
+  int max_num = 16;
+  int& max_w = A.max_w;
+  int& max_D = A.max_D;
   for (int t = 0; t < nt; ++t) {
     td[std::size_t(t)].cell_base = total_cells;
     td[std::size_t(t)].max_num = static_cast<std::int32_t>(tmax[std::size_t(t)]);
@@ -413,39 +447,61 @@ std::vector<TilesOut> match_tiles(Context& c, std::vector<Job>& jobs, const r3dg
     max_w = std::max(max_w, td[std::size_t(t)].w);
   }
   for (const Job& J : jobs) max_D = std::max(max_D, J.D);
-  if (max_num > 4096)
-    fail(R3DG_ERROR_INVALID_ARGUMENT, "stereo: more than 4096 labels per pixel are not supported on the GPU");
+  if (max_num > kMaxLabels)
+    fail(R3DG_ERROR_INVALID_ARGUMENT, "stereo: more than " + std::to_string(kMaxLabels) + " labels per pixel are not supported on the GPU");
   upload(c, dtiles, td);
+  // Group prefixes of the 8 paths (4 scanlines per wave), one upload.
+  std::vector<std::uint64_t> gp(8 * (std::size_t(nt) + 1), 0);
+  for (int p = 0; p < 8; ++p) {
+    std::uint64_t* g = gp.data() + std::size_t(p) * (std::size_t(nt) + 1);
+    for (int t = 0; t < nt; ++t)
+      g[t + 1] = g[t] + std::uint64_t((path_scanlines(p, td[std::size_t(t)].w, td[std::size_t(t)].h) + 3) / 4);
+  }
+  DBuf<std::uint64_t> dgp;
+  upload(c, dgp, gp);
+  A.S = DBuf<std::uint16_t>(c.arena, total_cells + kSPad);
+  DBuf<std::uint16_t>& S = A.S;
+  layout_timer.reset();
 
   // Aggregation: 8 path launches over all tiles.
-  DBuf<std::uint16_t> S(c.arena, std::max<std::uint64_t>(total_cells, 1));
-  for (int p = 0; p < 8; ++p) {
-    std::vector<std::uint64_t> gp(std::size_t(nt) + 1, 0);
-    for (int t = 0; t < nt; ++t)
-      gp[std::size_t(t) + 1] = gp[std::size_t(t)] +
-                               std::uint64_t((path_scanlines(p, td[std::size_t(t)].w, td[std::size_t(t)].h) + 3) / 4);
-    DBuf<std::uint64_t> dgp;
-    upload(c, dgp, gp);
-    launch_sgm_path(p, djobs.get(), dtiles.get(), dgp.get(), nt, gp.back(), row_start.get(), col_off.get(), S.get(),
-                    p > 0, max_num, c.stream);
-    check_launch();
+  {
+    const StageTimer timer(c, R3DG_STAGE_AGGREGATE);
+    for (int p = 0; p < 8; ++p) {
+      const std::size_t o8 = std::size_t(p) * (std::size_t(nt) + 1);
+      launch_sgm_path(p, djobs.get(), dtiles.get(), dgp.get() + o8, nt, gp[o8 + std::size_t(nt)], row_start.get(),
+                      pix.get(), S.get(), total_cells, p > 0, max_num, c.stream);
+      check_launch();
+    }
   }
+  c.agg_launches += 8;
+  c.agg_bytes += 30 * total_cells + 176 * total_px;
 
+  return A;
+}
+
+std::vector<TilesOut> match_tiles(Context& c, std::vector<Job>& jobs, const r3dg_options& o, SelectParams sp) {
+  const int nj = static_cast<int>(jobs.size());
+  Aggregation A = aggregate_jobs(c, jobs, o);
+  const int nt = static_cast<int>(A.td.size());
+  const std::uint64_t total_px = A.total_px;
   // Selection into per-tile maps.
   bool any_labels = false;
   for (const Job& J : jobs) any_labels = any_labels || J.labels;
   DBuf<float> tdisp(c.arena, std::max<std::uint64_t>(total_px, 1)), tconf(c.arena, std::max<std::uint64_t>(total_px, 1));
   DBuf<std::int16_t> tlab;
   if (any_labels) tlab = DBuf<std::int16_t>(c.arena, std::max<std::uint64_t>(total_px, 1));
-  launch_select(djobs.get(), dtiles.get(), drow_prefix.get(), nt, total_rows, row_start.get(), col_off.get(), S.get(),
-                sp, max_w, max_D, tdisp.get(), tconf.get(), tlab.get(), c.stream);
-  check_launch();
-  S.reset();
+  {
+    const StageTimer timer(c, R3DG_STAGE_SELECT);
+    launch_select(A.djobs.get(), A.dtiles.get(), A.drow_prefix.get(), nt, A.total_rows, A.row_start.get(),
+                  A.pix.get(), A.S.get(), sp, A.max_w, A.max_D, tdisp.get(), tconf.get(), tlab.get(), c.stream);
+    check_launch();
+  }
+  A.S.reset();
 
   std::vector<TilesOut> out(static_cast<std::size_t>(nj));
   for (int j = 0; j < nj; ++j) {
     const Job& J = jobs[std::size_t(j)];
-    const Grid& g = grids[std::size_t(j)];
+    const Grid& g = A.grids[std::size_t(j)];
     TilesOut& r = out[std::size_t(j)];
     r.disp = alloc_img<float>(c, J.base->w, J.base->h);
     if (J.confidence) r.conf = alloc_img<float>(c, J.base->w, J.base->h);
@@ -464,17 +520,20 @@ std::vector<TilesOut> match_tiles(Context& c, std::vector<Job>& jobs, const r3dg
     gd.width = J.base->w;
     gd.height = J.base->h;
     gd.blend = o.tile_blend;
-    gd.tile_base = tile_base[std::size_t(j)];
-    launch_blend(gd, dtiles.get(), tdisp.get(), tconf.get(), J.labels ? tlab.get() : nullptr, r.disp.p,
-                 J.confidence ? r.conf.p : nullptr, J.labels ? r.lab.p : nullptr, c.stream);
-    check_launch();
-    for (int t = tile_base[std::size_t(j)]; t < tile_base[std::size_t(j) + 1]; ++t) {
-      r.cells += tcells[std::size_t(t)];
-      r.tile_pixels += std::uint64_t(td[std::size_t(t)].w) * std::uint64_t(td[std::size_t(t)].h);
+    gd.tile_base = A.tile_base[std::size_t(j)];
+    {
+      const StageTimer timer(c, R3DG_STAGE_BLEND);
+      launch_blend(gd, A.dtiles.get(), tdisp.get(), tconf.get(), J.labels ? tlab.get() : nullptr, r.disp.p,
+                   J.confidence ? r.conf.p : nullptr, J.labels ? r.lab.p : nullptr, c.stream);
+      check_launch();
     }
-    r.tiles = tile_base[std::size_t(j) + 1] - tile_base[std::size_t(j)];
-    // The blend reads dcols / drows: keep them until the stream reaches it.
-    sync(c);
+    for (int t = A.tile_base[std::size_t(j)]; t < A.tile_base[std::size_t(j) + 1]; ++t) {
+      r.cells += A.tcells[std::size_t(t)];
+      r.tile_pixels += std::uint64_t(A.td[std::size_t(t)].w) * std::uint64_t(A.td[std::size_t(t)].h);
+    }
+    r.tiles = A.tile_base[std::size_t(j) + 1] - A.tile_base[std::size_t(j)];
+    // dcols / drows return to the arena here; later uses of the same memory
+    // are ordered after the blend on the context's stream.
   }
   return out;
 }
@@ -516,11 +575,14 @@ LevelOut match_level(Context& c, const LevelIn& in, const DImg<float>* coarse_le
   const bool check_lr = o.lr_max_diff >= 0;
   const bool full = check_lr && o.lr_full_match != 0;
 
+  std::optional<StageTimer> prep_timer(std::in_place, c, R3DG_STAGE_PREPARE);
   DBuf<std::int32_t> lrows = intervals(c, *in.mL, wl, h);
   DBuf<std::int32_t> rrows = intervals(c, *in.mR, wr, h);
   Ranges lranges, rranges;
-  if (coarse_left != nullptr)
+  if (coarse_left != nullptr) {
+    const StageTimer timer(c, R3DG_STAGE_RANGES);
     lranges = ranges_from_coarse(c, *coarse_left, wl, h, *in.mL, rrows.get(), in.dmin, in.nd, o, -1, false);
+  }
 
   std::vector<Job> jobs(1);
   jobs[0].base = in.L;
@@ -529,6 +591,7 @@ LevelOut match_level(Context& c, const LevelIn& in, const DImg<float>* coarse_le
   jobs[0].cr = census(c, *in.R);
   jobs[0].lrows = lrows.get();
   jobs[0].rrows = rrows.get();
+  jobs[0].rows4 = pack_rows(c, lrows.get(), rrows.get(), h);
   jobs[0].ranges = coarse_left != nullptr ? &lranges : nullptr;
   jobs[0].dmin = in.dmin;
   jobs[0].D = D;
@@ -547,8 +610,10 @@ LevelOut match_level(Context& c, const LevelIn& in, const DImg<float>* coarse_le
     mRv = mirror_u8(c, *in.mR);
     mrows_base = intervals(c, mRv, wr, h);
     mrows_partner = intervals(c, mLv, wl, h);
-    if (coarse_right != nullptr)
+    if (coarse_right != nullptr) {
+      const StageTimer timer(c, R3DG_STAGE_RANGES);
       rranges = ranges_from_coarse(c, *coarse_right, wr, h, *in.mR, lrows.get(), in.dmin, in.nd, o, 1, true);
+    }
     Job b;
     b.base = &mRi;
     b.partner = &mLi;
@@ -556,6 +621,7 @@ LevelOut match_level(Context& c, const LevelIn& in, const DImg<float>* coarse_le
     b.cr = census(c, mLi);
     b.lrows = mrows_base.get();
     b.rrows = mrows_partner.get();
+    b.rows4 = pack_rows(c, mrows_base.get(), mrows_partner.get(), h);
     b.ranges = coarse_right != nullptr ? &rranges : nullptr;
     b.dmin = in.dmin + wr - wl;
     b.D = D;
@@ -567,7 +633,9 @@ LevelOut match_level(Context& c, const LevelIn& in, const DImg<float>* coarse_le
   sp.uniqueness = o.uniqueness;
   sp.lr_max_diff = o.lr_max_diff;
   sp.diagonal_lr = check_lr && !full ? 1 : 0;
+  prep_timer.reset();
   std::vector<TilesOut> maps = match_tiles(c, jobs, o, sp);
+  const StageTimer filter_timer(c, R3DG_STAGE_FILTER);
 
   LevelOut out;
   out.left = std::move(maps[0]);
@@ -624,12 +692,14 @@ MatchResult match_stereo(Context& c, const DImg<std::uint16_t>& left, const DImg
   const int dmax = dmin + nd - 1;
   std::vector<DImg<std::uint16_t>> lp, rp;
   std::vector<DImg<std::uint8_t>> lvp, rvp;
+  std::optional<StageTimer> pyr_timer(std::in_place, c, R3DG_STAGE_PYRAMID);
   for (int l = 1; l <= levels; ++l) {
     lp.push_back(pyramid_u16(c, l == 1 ? left : lp.back()));
     rp.push_back(pyramid_u16(c, l == 1 ? right : rp.back()));
     if (!lvalid.empty()) lvp.push_back(pyramid_u8(c, l == 1 ? lvalid : lvp.back(), true));
     if (!rvalid.empty()) rvp.push_back(pyramid_u8(c, l == 1 ? rvalid : rvp.back(), true));
   }
+  pyr_timer.reset();
   MatchResult res;
   res.levels = levels;
   LevelOut coarse;
@@ -796,6 +866,10 @@ int32_t r3dg_last_error(const r3dg_ctx* h, char* buf, size_t size) {
 
 int32_t r3dg_global_error(char* buf, size_t size) { return copy_error(g_error, g_status, buf, size); }
 
+int32_t r3dg_set_profiling(r3dg_ctx* h, int32_t enabled) {
+  return with_ctx(h, [&](Context& c) { c.profiling = enabled != 0 ? 1 : 0; });
+}
+
 int32_t r3dg_cancel(r3dg_ctx* h, int32_t requested) {
   if (h == nullptr) return R3DG_ERROR_INVALID_ARGUMENT;
   reinterpret_cast<Context*>(h)->cancel.store(requested != 0 ? 1 : 0);
@@ -906,74 +980,14 @@ int32_t r3dg_cost_volume(r3dg_ctx* h, r3dg_image left, r3dg_image right, int32_t
     jobs[0].cr = census(c, R);
     jobs[0].lrows = lrows.get();
     jobs[0].rrows = rrows.get();
+    jobs[0].rows4 = pack_rows(c, lrows.get(), rrows.get(), L.h);
     jobs[0].ranges = has ? &rg : nullptr;
     jobs[0].dmin = disparity_min;
     jobs[0].D = D;
-    // The volume itself: run the layout and the 8 paths, keep S.
-    const auto lut = p2_table(o);
-    JobDesc d{};
-    d.img = L.p;
-    d.img_stride = L.s;
-    d.cl = jobs[0].cl.p;
-    d.cl_stride = jobs[0].cl.s;
-    d.cr = jobs[0].cr.p;
-    d.cr_stride = jobs[0].cr.s;
-    d.lrows = lrows.get();
-    d.rrows = rrows.get();
-    d.lo = has ? rg.lo.p : nullptr;
-    d.num = has ? rg.num.p : nullptr;
-    d.r_stride = has ? rg.lo.s : 0;
-    d.width = L.w;
-    d.height = L.h;
-    d.pwidth = R.w;
-    d.dmin = disparity_min;
-    d.D = D;
-    d.p1 = o.p1;
-    d.invalid_cost = o.invalid_cost;
-    d.confidence = 1;
-    std::memcpy(d.lut, lut.data(), sizeof(d.lut));
-    DBuf<JobDesc> dj;
-    upload(c, dj, std::vector<JobDesc>{d});
-    TileDesc t{};
-    t.x0 = 0;
-    t.y0 = 0;
-    t.w = L.w;
-    t.h = L.h;
-    DBuf<TileDesc> dt;
-    upload(c, dt, std::vector<TileDesc>{t});
-    DBuf<std::uint64_t> rp;
-    upload(c, rp, std::vector<std::uint64_t>{0});
-    DBuf<std::uint32_t> col_off(c.arena, L.pixels()), row_max(c.arena, std::size_t(L.h)), tmax(c.arena, 1);
-    DBuf<std::uint64_t> row_cells(c.arena, std::size_t(L.h)), row_start(c.arena, std::size_t(L.h)), tc(c.arena, 1);
-    launch_tile_row_scan(dj.get(), dt.get(), rp.get(), 1, std::uint64_t(L.h), col_off.get(), row_cells.get(),
-                         row_max.get(), c.stream);
-    check_launch();
-    launch_tile_cells(dt.get(), 1, row_cells.get(), row_max.get(), row_start.get(), tc.get(), tmax.get(), c.stream);
-    check_launch();
-    std::uint32_t max_num = 0;
-    hip_check(hipMemcpyAsync(&max_num, tmax.get(), 4, hipMemcpyDeviceToHost, c.stream), "download");
-    sync(c);
-    if (max_num > 4096)
-      fail(R3DG_ERROR_INVALID_ARGUMENT, "stereo: more than 4096 labels per pixel are not supported on the GPU");
-    DBuf<std::uint16_t> S;
-    std::uint16_t* dst = nullptr;
-    if (out_memory == R3DG_DEVICE) {
-      dst = out;
-    } else {
-      S = DBuf<std::uint16_t>(c.arena, std::max<std::uint64_t>(out_cells, 1));
-      dst = S.get();
-    }
-    for (int p = 0; p < 8; ++p) {
-      const std::uint64_t groups = std::uint64_t((path_scanlines(p, L.w, L.h) + 3) / 4);
-      DBuf<std::uint64_t> gp;
-      upload(c, gp, std::vector<std::uint64_t>{0, groups});
-      launch_sgm_path(p, dj.get(), dt.get(), gp.get(), 1, groups, row_start.get(), col_off.get(), dst, p > 0,
-                      static_cast<int>(std::max<std::uint32_t>(max_num, 16)), c.stream);
-      check_launch();
-      sync(c);
-    }
-    if (out_memory != R3DG_DEVICE)
-      hip_check(hipMemcpyAsync(out, dst, out_cells * 2, hipMemcpyDeviceToHost, c.stream), "download");
+    Aggregation A = aggregate_jobs(c, jobs, one);
+    if (A.total_cells != out_cells) fail(R3DG_ERROR_INTERNAL, "cost_volume: cell count mismatch");
+    const hipMemcpyKind kind = out_memory == R3DG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    if (out_cells > 0) hip_check(hipMemcpyAsync(out, A.S.get(), out_cells * 2, kind, c.stream), "download");
     sync(c);
   });
 }
@@ -992,6 +1006,11 @@ int32_t r3dg_match(r3dg_ctx* h, r3dg_image left, r3dg_image right, r3dg_image le
     DImg<std::uint8_t> mR = input_img<std::uint8_t>(c, right_valid, "right_valid");
     if (view_empty(disparity)) fail(R3DG_ERROR_INVALID_ARGUMENT, "match: empty disparity output");
     const bool want_debug = debug != nullptr && (!view_empty(debug->labels) || !view_empty(debug->lr_mask));
+    c.agg_launches = 0;
+    c.agg_bytes = 0;
+    double stage_seconds[R3DG_STAGES] = {};
+    c.collect(stage_seconds, R3DG_STAGES);  // drop marks of earlier calls
+    std::fill(stage_seconds, stage_seconds + R3DG_STAGES, 0.0);
     MatchResult r = match_stereo(c, L, R, mL, mR, disparity_min, num_disparities, o, want_debug);
     const TilesOut& m = r.final.left;
     output_img(c, disparity, m.disp.p, m.disp.w, m.disp.h, m.disp.s, "disparity");
@@ -1011,6 +1030,12 @@ int32_t r3dg_match(r3dg_ctx* h, r3dg_image left, r3dg_image right, r3dg_image le
       info->cells = r.cells;
       info->tile_pixels = r.tile_pixels;
       info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      info->aggregate_launches = c.agg_launches;
+      info->aggregate_bytes = c.agg_bytes;
+      c.collect(stage_seconds, R3DG_STAGES);
+      for (int k = 0; k < R3DG_STAGES; ++k) info->stage_seconds[k] = stage_seconds[k];
+    } else {
+      c.collect(stage_seconds, R3DG_STAGES);
     }
   });
 }

@@ -11,6 +11,7 @@
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
+#include <vector>
 
 #include "r3dg.h"
 
@@ -158,7 +159,80 @@ struct Context {
   std::atomic<int> cancel{0};
   std::string error;
   int32_t error_status = R3DG_OK;
+  // Profiling: HIP event pairs around stages, read back at the end of a call.
+  int profiling = 0;
+  struct Mark {
+    int stage;
+    hipEvent_t start, stop;
+  };
+  std::vector<Mark> marks;
+  std::vector<hipEvent_t> event_pool;
+  std::uint64_t agg_launches = 0;
+  std::uint64_t agg_bytes = 0;
+
   explicit Context(std::uint64_t budget) : arena(budget) {}
+  ~Context() {
+    for (auto& m : marks) {
+      event_pool.push_back(m.start);
+      event_pool.push_back(m.stop);
+    }
+    for (hipEvent_t e : event_pool) (void)hipEventDestroy(e);
+  }
+  hipEvent_t event() {
+    if (!event_pool.empty()) {
+      hipEvent_t e = event_pool.back();
+      event_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+  }
+  // Adds the recorded stage durations to `seconds` (the stream must be idle).
+  void collect(double* seconds, int stages) {
+    for (auto& m : marks) {
+      float ms = 0.0F;
+      if (hipEventElapsedTime(&ms, m.start, m.stop) == hipSuccess && m.stage >= 0 && m.stage < stages)
+        seconds[m.stage] += 1e-3 * ms;
+      event_pool.push_back(m.start);
+      event_pool.push_back(m.stop);
+    }
+    marks.clear();
+  }
+};
+
+// Records a stage's start and stop on the context's stream when profiling.
+class StageTimer {
+ public:
+  StageTimer(Context& c, int stage) : c_(c), stage_(stage) {
+    if (c_.profiling != 0) {
+      start_ = c_.event();
+      hip_check(hipEventRecord(start_, c_.stream), "hipEventRecord");
+    }
+  }
+  StageTimer(const StageTimer&) = delete;
+  StageTimer& operator=(const StageTimer&) = delete;
+  ~StageTimer() {
+    if (start_ == nullptr) return;
+    hipEvent_t stop = nullptr;
+    try {
+      stop = c_.event();
+    } catch (...) {
+      c_.event_pool.push_back(start_);
+      return;
+    }
+    if (hipEventRecord(stop, c_.stream) == hipSuccess) {
+      c_.marks.push_back({stage_, start_, stop});
+    } else {
+      c_.event_pool.push_back(start_);
+      c_.event_pool.push_back(stop);
+    }
+  }
+
+ private:
+  Context& c_;
+  int stage_;
+  hipEvent_t start_ = nullptr;
 };
 
 }  // namespace r3dg

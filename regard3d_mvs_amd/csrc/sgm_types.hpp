@@ -10,24 +10,37 @@
 //
 // HBM layout of the aggregated costs S (uint16): every tile owns a contiguous
 // packed volume at cell_base; inside it, tile row y starts at
-// row_start[row_base + y] and pixel x of that row at col_off[px_base + y*w + x]
-// (the TileLabels layout of stereo_matcher.cpp:172-224).  Label counts are
-// multiples of 16, so every pixel's labels start on a 32-byte boundary.
+// row_start[row_base + y] and pixel x of that row at the cell offset held in
+// its pixel record pix[px_base + y*w + x] (the TileLabels layout of
+// stereo_matcher.cpp:172-224).  Label counts are multiples of 16, so every
+// pixel's labels start on a 32-byte boundary.
+//
+// Pixel record (uint64): bits 0-31 cell offset inside the tile row, bits 32-47
+// first label `lo`, bits 48-63 label count `num`.
 #pragma once
+
+#include <hip/hip_runtime.h>
 
 #include <cstdint>
 
 namespace r3dg {
 
 constexpr int kLabelAlign = 16;
+constexpr int kMaxLabels = 4000;  // per pixel: two LDS rows per scanline must fit 64 KiB
+
+__host__ __device__ constexpr std::uint64_t pix_record(std::uint32_t off, int lo, int num) {
+  return std::uint64_t(off) | (std::uint64_t(std::uint16_t(lo)) << 32) | (std::uint64_t(std::uint16_t(num)) << 48);
+}
+__host__ __device__ constexpr std::uint32_t pix_off(std::uint64_t r) { return std::uint32_t(r); }
+__host__ __device__ constexpr int pix_lo(std::uint64_t r) { return int((r >> 32) & 0xFFFFU); }
+__host__ __device__ constexpr int pix_num(std::uint64_t r) { return int((r >> 48) & 0xFFFFU); }
 constexpr int kMaxLevels = 13;
 
 struct JobDesc {
   const std::uint16_t* img;  // base image (P2 intensities), width x height
   const std::uint64_t* cl;   // base Census
   const std::uint64_t* cr;   // partner Census, pwidth x height
-  const std::int32_t* lrows; // [height][2] valid interval of the base rows
-  const std::int32_t* rrows; // [height][2] valid interval of the partner rows
+  const std::int32_t* rows4; // [height][4] valid columns of the row: base first, last; partner first, last
   const std::uint16_t* lo;   // per-pixel label ranges (nullptr: full range)
   const std::uint16_t* num;
   std::int64_t img_stride;   // elements

@@ -12,7 +12,7 @@ import re
 
 import numpy as np
 
-from ._abi import (R3DG_DEVICE, R3DG_HOST, CImage, CMatchDebug, CMatchInfo, DisparityMap, Status, StereoOptions,
+from ._abi import (R3DG_DEVICE, R3DG_HOST, STAGES, CImage, CMatchDebug, CMatchInfo, DisparityMap, Status, StereoOptions,
                    align16)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -208,7 +208,12 @@ class Matcher:
         self._check(lib().r3dg_match(self._ctx, lv, rv, mlv, mrv, int(dmin), int(nd), ctypes.byref(o), dv, cv,
                                      ctypes.byref(dbg) if dbg is not None else None, ctypes.byref(info)))
         return DisparityMap(disp, conf, info.disparity_min, info.num_disparities, info.levels, int(info.cells),
-                            int(info.tile_pixels), info.tiles_level0, labels, lr, info.seconds)
+                            int(info.tile_pixels), info.tiles_level0, labels, lr, info.seconds,
+                            int(info.aggregate_launches), int(info.aggregate_bytes),
+                            {k: float(info.stage_seconds[i]) for i, k in enumerate(STAGES)})
+
+    def set_profiling(self, enabled=True):
+        self._check(lib().r3dg_set_profiling(self._ctx, int(bool(enabled))))
 
     # remove_speckles (stereo_matcher.hpp:89): returns a filtered copy
     def remove_speckles(self, d, min_size, max_diff):
