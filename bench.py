@@ -40,6 +40,21 @@ def percentile(v, q):
     return float(np.percentile(np.asarray(v), q)) if len(v) else 0.0
 
 
+def measured_traffic(config):
+    """HBM bytes per path launch from the committed FETCH_SIZE / WRITE_SIZE profile
+    of this workload (None if there is none)."""
+    if config != 2:
+        return None
+    path = os.path.join(ROOT, "profiles", "k_path_24mp_512d_pmc.md")
+    try:
+        for line in open(path):
+            if line.strip().startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(pair, options, rows, threads):
     """The oracle on rows [0, rows) of the pair; rate scaled to whole pairs."""
     from oracle import oracle as O
@@ -116,7 +131,7 @@ def main() -> int:
 
     achieved = agg_b / agg_s if agg_s > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": None,
+                "frac": achieved / HBM_PEAK, "traffic": measured_traffic(args.config),
                 "kernel": "k_path (8 SGM path scans per level)",
                 "algorithmic_bytes_per_launch": agg_b / max(launches, 1),
                 "avg_launch_ms": 1e3 * agg_s / max(launches, 1)}

@@ -20,6 +20,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libr3dg.so")
+API_LIB = os.path.join(HERE, "libr3d_stereo.so")          # r3d:: C++ API over the C ABI
+API_TEST = os.path.join(ROOT, "tests", "cpp", "_build", "test_stereo_api")
 SOURCES = ["kernels_image.hip", "kernels_sgm.hip", "matcher.cpp"]
 ARCH = os.environ.get("R3DG_ARCH", "gfx950")
 
@@ -56,9 +58,34 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    _build_api()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB}, {API_LIB}, {API_TEST}")
     return LIB
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+
+
+def _build_api() -> None:
+    """libr3d_stereo.so (host C++ only, over libr3dg.so) and its C++ test program."""
+    inc = os.path.join(ROOT, "include")
+    cxx = ["g++", "-O2", "-std=c++20", "-fPIC", "-Wall", "-Wextra", "-I", inc]
+    tmp = API_LIB + ".tmp"
+    _run(cxx + ["-shared", "-o", tmp, os.path.join(CSRC, "api", "r3d_stereo.cpp"), "-L", HERE, "-lr3dg",
+                "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, API_LIB)
+    tdir = os.path.dirname(API_TEST)
+    os.makedirs(tdir, exist_ok=True)
+    cpp = os.path.join(ROOT, "tests", "cpp")
+    oracle_src = [os.path.join(ROOT, "oracle", "oracle.cpp")]
+    _run(cxx + ["-ffp-contract=off", "-pthread", "-o", API_TEST + ".tmp", os.path.join(cpp, "test_stereo_api.cpp"),
+                os.path.join(cpp, "reference_shim.cpp")] + oracle_src +
+         ["-L", HERE, "-lr3d_stereo", "-lr3dg", f"-Wl,-rpath,{HERE}"])
+    os.replace(API_TEST + ".tmp", API_TEST)
 
 
 if __name__ == "__main__":
