@@ -61,11 +61,24 @@ void launch_tile_row_scan(const JobDesc* jobs, const TileDesc* tiles, const std:
 // tile_max[t] = largest label count of the tile.
 void launch_tile_cells(const TileDesc* tiles, int ntiles, const std::uint64_t* row_cells, const std::uint32_t* row_max,
                        std::uint64_t* row_start, std::uint64_t* tile_cells, std::uint32_t* tile_max, hipStream_t s);
+// Path kernel geometry: a wave64 scans kScanPerWave scanlines with
+// kLanesPerScan lanes each (group prefixes count waves).
+constexpr int kLanesPerScan = 8;
+constexpr int kScanPerWave = 64 / kLanesPerScan;
+// Matching costs (u8) of every tile cell into C (same cell indexing as S).
+void launch_cost(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* row_prefix, int ntiles,
+                 std::uint64_t total_rows, const std::uint64_t* row_start, const std::uint64_t* pix, std::uint8_t* C,
+                 hipStream_t s);
 // One SGM path over every tile (path index 0..7, see kernels_sgm.hip).
+// `scratch` holds path_scratch_elems(groups, max_num) uint16 (rows of pixels
+// wider than the LDS rows); it may be null when that is 0.
+std::size_t path_scratch_elems(std::uint64_t groups, int max_num);
+int path_scratch_row(int max_num);
 void launch_sgm_path(int path, const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* group_prefix, int ntiles,
                      std::uint64_t groups, const std::uint64_t* row_start, const std::uint64_t* pix,
-                     std::uint16_t* S, std::uint64_t total_cells, bool accumulate, int max_num, hipStream_t s);
-// Extra uint16 elements the volume S needs past its last cell (prefetch reads).
+                     const std::uint8_t* C, std::uint16_t* S, std::uint16_t* scratch, int max_num, bool accumulate,
+                     hipStream_t s);
+// Extra elements S and C hold past their last cell.
 constexpr std::size_t kSPad = 64;
 // WTA + sub-pixel + checks of every tile row into per-tile-pixel results.
 void launch_select(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* row_prefix, int ntiles,

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -178,179 +179,419 @@ __device__ __forceinline__ void scan_start(int s, int w, int h, int& x, int& y, 
   }
 }
 
-// Min over the 16 lanes of a DPP row, broadcast to all of them (row_ror).
-__device__ __forceinline__ std::uint32_t row_min16(std::uint32_t v) {
-  v = min(v, static_cast<std::uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x128, 0xF, 0xF, false)));
-  v = min(v, static_cast<std::uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x124, 0xF, 0xF, false)));
-  v = min(v, static_cast<std::uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x122, 0xF, 0xF, false)));
-  v = min(v, static_cast<std::uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x121, 0xF, 0xF, false)));
+// ------------------------------------------------------------------ costs
+// Matching costs of every tile cell (stereo_matcher.cpp:233-247, census_costs):
+// the Hamming distance of the base Census and the partner Census at column
+// X - dmin - lo - j when that column lies in the partner row's valid interval,
+// invalid_cost when it does not, 0 for base pixels outside the base row's
+// valid interval.  One block per tile row; a thread per 16-label chunk finds
+// its pixel by binary search over the row's cell offsets and writes 16 bytes.
+__global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, const TileDesc* __restrict__ tiles,
+                                              const std::uint64_t* __restrict__ row_prefix, int ntiles,
+                                              const std::uint64_t* __restrict__ row_start,
+                                              const std::uint64_t* __restrict__ pix, std::uint8_t* __restrict__ C) {
+  const std::uint64_t r = blockIdx.x;
+  const int t = find_tile(row_prefix, ntiles, r);
+  const TileDesc T = tiles[t];
+  const JobDesc& J = jobs[T.job];
+  const int y = static_cast<int>(r - row_prefix[t]);
+  const int Y = T.y0 + y;
+  const auto rec = gptr(pix + T.px_base + std::uint64_t(y) * T.w);
+  const std::uint64_t last = rec[T.w - 1];
+  const std::uint32_t chunks = (pix_off(last) + static_cast<std::uint32_t>(pix_num(last))) >> 4;
+  const auto rows = gptr(J.rows4) + 4 * Y;
+  const int bfirst = rows[0], blast = rows[1], pfirst = rows[2], plast = rows[3];
+  const auto cl = gptr(J.cl) + std::size_t(Y) * J.cl_stride;
+  const auto cr = gptr(J.cr) + std::size_t(Y) * J.cr_stride;
+  const int pw1 = J.pwidth - 1;
+  const std::uint32_t invalid = static_cast<std::uint32_t>(J.invalid_cost);
+  std::uint8_t* out = C + T.cell_base + row_start[T.row_base + y];
+  for (std::uint32_t k = threadIdx.x; k < chunks; k += blockDim.x) {
+    // Every pixel holds at least one chunk, so the pixel of chunk k is at most k.
+    int xa = 0, xb = min(static_cast<int>(k), T.w - 1);
+    while (xa < xb) {
+      const int mid = (xa + xb + 1) >> 1;
+      if (pix_off(rec[mid]) <= 16 * k) {
+        xa = mid;
+      } else {
+        xb = mid - 1;
+      }
+    }
+    const std::uint64_t pr = rec[xa];
+    const int X = T.x0 + xa;
+    const int jb = static_cast<int>(16 * k - pix_off(pr));
+    const int xr0 = X - J.dmin - pix_lo(pr) - jb;
+    const bool lvalid = X >= bfirst && X <= blast;
+    const std::uint64_t c = cl[X];
+    std::uint32_t w[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      std::uint32_t v = 0;
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int xr = xr0 - 4 * q - b;
+        std::uint32_t cost = xr >= pfirst && xr <= plast ? static_cast<std::uint32_t>(__popcll(c ^ cr[min(max(xr, 0), pw1)]))
+                                                         : invalid;
+        cost = lvalid ? cost : 0U;
+        v |= cost << (8 * b);
+      }
+      w[q] = v;
+    }
+    *reinterpret_cast<uint4*>(out + 16 * std::size_t(k)) = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+}
+
+// ------------------------------------------------------------------ path scans
+// Packed pairs of uint16 path costs: v_pk_* instructions process two labels.
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ u16x2 as_pair(std::uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+__device__ __forceinline__ std::uint32_t as_u32(u16x2 v) { return __builtin_bit_cast(std::uint32_t, v); }
+
+// Min over the 8 lanes of a scanline (quad swaps, then the half-row mirror).
+__device__ __forceinline__ std::uint32_t min8(std::uint32_t v) {
+  v = min(v, static_cast<std::uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0xB1, 0xF, 0xF, false)));
+  v = min(v, static_cast<std::uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x4E, 0xF, 0xF, false)));
+  v = min(v, static_cast<std::uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x141, 0xF, 0xF, false)));
   return v;
 }
 
-// Per-step values of one scanline pixel that do not depend on the recursion.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// Loads and stores at a byte offset from a wave-uniform base (SGPR base +
+// 32-bit VGPR offset addressing).
+template <typename T>
+__device__ __forceinline__ T ldb(const void* base, std::uint32_t off) {
+  const auto p = (const __attribute__((address_space(1))) char*)(base);
+  return *reinterpret_cast<const __attribute__((address_space(1))) T*>(p + off);
+}
+template <typename T>
+__device__ __forceinline__ void stb(void* base, std::uint32_t off, T v) {
+  const auto p = (__attribute__((address_space(1))) char*)(base);
+  *reinterpret_cast<__attribute__((address_space(1))) T*>(p + off) = v;
+}
+
+// s_waitcnt immediate (gfx9 encoding) that waits for every vector memory
+// operation and nothing else: vmcnt 0, expcnt 7, lgkmcnt 15.
+constexpr int kVmcnt0 = 0x0F70;
+constexpr int kWaves = 4;                     // waves per block
+constexpr int kNarrow = 96;                   // labels whose path costs stay in LDS
+constexpr int kPad = 12;                      // sentinel slots before label 0 and after the last label
+constexpr int kRow = kPad + kNarrow + kPad;   // uint16 slots of an LDS row
+
+// Label segments of a pixel: segment 0 holds labels [0, 32), lane l holds
+// four of them from 4l; segment s >= 1 holds [32 + 64 (s - 1), 32 + 64 s),
+// lane l eight of them from 32 + 64 (s - 1) + 8l.  A lane's labels lie all
+// inside or all outside a pixel's range (label counts are multiples of 16).
+template <int P>
+__device__ __forceinline__ int seg_first(int s, int l8) {
+  return P == 4 ? 4 * l8 : 32 + 64 * (s - 1) + 8 * l8;
+}
+__device__ __forceinline__ int segments(int num) { return num <= 32 ? 1 : 1 + (num - 32 + 63) / 64; }
+
+// Costs (P bytes) and S (P uint16) of one lane in one segment.
+template <int P>
+struct SegIn {
+  std::uint32_t c[P / 4];
+  std::uint32_t s[P / 2];
+};
+
+// Per-step values of one scanline pixel, loaded two steps ahead.
 struct StepIn {
   std::uint64_t rec;  // pixel record
-  std::uint64_t rs;   // row start of the pixel's tile row
-  std::uint64_t cl;   // base Census
-  int4 rows;          // valid columns of the row (base first/last, partner first/last)
-  int ip;             // base intensity
+  std::uint32_t rs;   // cell offset of the pixel's tile row
+  std::uint32_t ip;   // base intensity
+};
+// Segments 0 and 1 of a pixel, loaded one step ahead.
+struct StepSegs {
+  SegIn<4> a;
+  SegIn<8> b;
 };
 
-// Labels j = lane + 16 m of the first kPre chunks, loaded one step ahead.
-template <int kPre>
-struct StepLanes {
-  std::uint32_t s[kPre];   // current S (accumulating paths)
-  std::uint64_t cr[kPre];  // partner Census at column X - dmin - lo - j (clamped into the image)
-};
-
-// One path direction over every tile.  The loop over the scanline's pixels is
-// software-pipelined: the pixel records, Census and row data of step i + 2
-// and the S / partner-Census values of step i + 1 are in flight while step i
-// computes, with no branch around them, so the compiler's wait counters stay
-// exact.  Steps past the end of a scanline (scanlines of a wave differ in
-// length) and label chunks past a pixel's range run with clamped loads and
-// send their stores to a per-lane dummy slot past the end of S.
-template <int DX, int DY, int kPre, bool kAcc>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_path(const JobDesc* __restrict__ jobs, const TileDesc* __restrict__ tiles,
-                                              const std::uint64_t* __restrict__ gprefix, int ntiles,
-                                              std::uint64_t groups, const std::uint64_t* __restrict__ row_start,
-                                              const std::uint64_t* __restrict__ pix, std::uint16_t* __restrict__ S,
-                                              std::uint64_t total_cells, int max_num) {
-  extern __shared__ std::uint16_t lds[];
-  const int waves = blockDim.x >> 6;
+// One path direction over every tile.  A wave scans 8 adjacent scanlines of a
+// tile, 8 lanes per scanline, a lane holding 4 or 8 consecutive labels of a
+// segment as packed uint16 pairs.  Recursion (stereo_matcher.cpp:258-330):
+//   L(j) = C(j) + min(Lp(k), sat(min(Lp(k-1), Lp(k+1)) + P1), sat(mp + P2)) - mp
+// with k = j + lo - lo_prev, mp the predecessor's minimum, Lp = 0xFFFF outside
+// its labels and L = C at the first pixel.  The predecessor's path costs sit
+// in a row with 0xFFFF sentinels around its labels, in LDS for pixels of at
+// most kNarrow labels and in a global scratch row for wider ones; a lane reads
+// its window of P + 2 values as aligned words (the window clamped into the
+// sentinels) and shifts them into place.  Segments 0 and 1 use data prefetched
+// a step ahead (pixel records two steps ahead); segments from 2 on (pixels
+// wider than kNarrow) are loaded in the step, one segment ahead.
+template <int DX, int DY, bool kAcc>
+__global__ __launch_bounds__(64 * kWaves) void k_path(
+    const JobDesc* __restrict__ jobs, const TileDesc* __restrict__ tiles, const std::uint64_t* __restrict__ gprefix,
+    int ntiles, std::uint64_t groups, const std::uint64_t* __restrict__ row_start,
+    const std::uint64_t* __restrict__ pix, const std::uint8_t* __restrict__ C, std::uint16_t* __restrict__ S,
+    std::uint16_t* __restrict__ scratch, int scratch_row) {
+  __shared__ __attribute__((aligned(16))) std::uint16_t lut[kWaves][256];
+  __shared__ __attribute__((aligned(16))) std::uint32_t rows[kWaves][kScanPerWave][2][kRow / 2];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63, grp = lane >> 4, l16 = lane & 15;
-  const std::uint64_t gid = std::uint64_t(blockIdx.x) * waves + wave;
+  const int lane = threadIdx.x & 63, grp = lane / kLanesPerScan, l8 = lane % kLanesPerScan;
+  const std::uint64_t gid = std::uint64_t(blockIdx.x) * kWaves + wave;
   const bool wave_active = gid < groups;
   const int t = find_tile(gprefix, ntiles, wave_active ? gid : gprefix[ntiles] - 1);
   const TileDesc T = tiles[t];
   const JobDesc& J = jobs[T.job];
-  std::uint16_t* my_lut = lds + wave * 256;  // P2 table of the wave's job
-  for (int k = lane; k < 256; k += 64) my_lut[k] = J.lut[k];
+  for (int k = lane; k < 256; k += 64) lut[wave][k] = J.lut[k];
+  if (l8 < kPad / 2) {  // lead sentinels of both rows
+    rows[wave][grp][0][l8] = 0xFFFFFFFFU;
+    rows[wave][grp][1][l8] = 0xFFFFFFFFU;
+  }
   __syncthreads();
   if (!wave_active) return;
   const int ns = scanlines<DX, DY>(T.w, T.h);
-  const int s0 = static_cast<int>(gid - gprefix[t]) * 4;
-  const int s = min(s0 + grp, ns - 1);     // idle groups shadow the last scanline
+  const int s0 = static_cast<int>(gid - gprefix[t]) * kScanPerWave;
+  const int sl = min(s0 + grp, ns - 1);  // idle groups shadow the last scanline
   const bool grp_active = s0 + grp < ns;
   int x0, y0, len;
-  scan_start<DX, DY>(s, T.w, T.h, x0, y0, len);
-  // Steps of the wave: the longest of its scanlines.
-  int steps = grp_active ? len : 0;
-  steps = max(steps, __shfl_xor(steps, 16));
-  steps = max(steps, __shfl_xor(steps, 32));
-  steps = __builtin_amdgcn_readfirstlane(steps);
+  scan_start<DX, DY>(sl, T.w, T.h, x0, y0, len);
+  auto wave_max = [](int v) {
+    v = max(v, __shfl_xor(v, 8));
+    v = max(v, __shfl_xor(v, 16));
+    v = max(v, __shfl_xor(v, 32));
+    return __builtin_amdgcn_readfirstlane(v);
+  };
+  const int steps = wave_max(grp_active ? len : 0);
 
-  // LDS row of a scanline: the labels plus room for the masked-off chunks of a
-  // 4-chunk batch (their writes land past the row's labels and are never read).
-  const int rowlen = (max_num > 16 * kPre ? max_num : 16 * kPre) + 48;
-  std::uint16_t* buf = lds + waves * 256 + std::size_t(wave * 4 + grp) * 2 * rowlen;
-  const std::uint32_t P1 = static_cast<std::uint32_t>(J.p1);
-  const std::uint32_t invalid = static_cast<std::uint32_t>(J.invalid_cost);
-  const int pw1 = J.pwidth - 1;
-  const auto tpix = gptr(pix + T.px_base);
-  const auto trs = gptr(row_start + T.row_base);
-  const auto gcl = gptr(J.cl);
-  const auto gcr = gptr(J.cr);
-  const auto gimg = gptr(J.img);
-  const auto grows = gptr(J.rows4);
-  const auto gS = gptr_mut(S);
-  const auto dummy = gS + total_cells + lane;  // inside the padding of S
+  const std::uint32_t P1x2 = static_cast<std::uint32_t>(J.p1) * 0x10001U;
+  const std::uint64_t* tpix = pix + T.px_base;
+  const std::uint64_t* trs = row_start + T.row_base;
+  const std::uint16_t* img = J.img;
+  const int img_stride = static_cast<int>(J.img_stride);
+  const std::uint8_t* Ct = C + T.cell_base;
+  std::uint16_t* St = S + T.cell_base;
+  // This scanline's two scratch rows (same layout as the LDS rows).
+  __attribute__((address_space(1))) std::uint32_t* srow =
+      scratch != nullptr
+          ? reinterpret_cast<__attribute__((address_space(1))) std::uint32_t*>(
+                gptr_mut(scratch + (gid * kScanPerWave + grp) * 2 * std::uint64_t(scratch_row)))
+          : nullptr;
+  const int swords = scratch_row / 2;
+  if (srow != nullptr && l8 < kPad / 2) {
+    srow[l8] = 0xFFFFFFFFU;
+    srow[swords + l8] = 0xFFFFFFFFU;
+  }
 
   auto load_in = [&](int i, StepIn& u) {
     i = min(i, len - 1);
     const int x = x0 + i * DX, y = y0 + i * DY;
-    const int X = T.x0 + x, Y = T.y0 + y;
-    u.rec = tpix[std::int64_t(y) * T.w + x];
-    u.rs = trs[y];
-    u.cl = gcl[Y * J.cl_stride + X];
-    u.rows = make_int4(grows[4 * Y], grows[4 * Y + 1], grows[4 * Y + 2], grows[4 * Y + 3]);
-    u.ip = gimg[Y * J.img_stride + X];
+    u.rec = ldb<std::uint64_t>(tpix, static_cast<std::uint32_t>(y * T.w + x) * 8U);
+    u.rs = ldb<std::uint32_t>(trs, static_cast<std::uint32_t>(y) * 8U);
+    u.ip = ldb<std::uint16_t>(img, static_cast<std::uint32_t>((T.y0 + y) * img_stride + T.x0 + x) * 2U);
   };
-  auto load_lanes = [&](int i, const StepIn& u, StepLanes<kPre>& v) {
-    i = min(i, len - 1);
-    const int X = T.x0 + x0 + i * DX, Y = T.y0 + y0 + i * DY;
-    const std::uint64_t cell = T.cell_base + u.rs + pix_off(u.rec);
-    const int xr0 = X - J.dmin - pix_lo(u.rec) - l16;
-    const auto crow = gcr + Y * J.cr_stride;
-#pragma unroll
-    for (int m = 0; m < kPre; ++m) {
-      // Chunks past the pixel's labels read neighbouring (padded) cells and a
-      // clamped column; their values are never used.
-      if constexpr (kAcc) v.s[m] = gS[cell + l16 + 16 * m];
-      v.cr[m] = crow[min(max(xr0 - 16 * m, 0), pw1)];
+  // Costs and S of segment s (lanes past the pixel's labels read its last ones;
+  // those values are never used).
+  auto load_seg4 = [&](const StepIn& u, SegIn<4>& d) {
+    const std::uint32_t cell = u.rs + pix_off(u.rec);
+    const std::uint32_t j0 = static_cast<std::uint32_t>(min(4 * l8, pix_num(u.rec) - 4));
+    d.c[0] = ldb<std::uint32_t>(Ct, cell + j0);
+    if constexpr (kAcc) {
+      const u32x2 v = ldb<u32x2>(St, (cell + j0) * 2U);
+      d.s[0] = v.x;
+      d.s[1] = v.y;
     }
+  };
+  auto load_seg8 = [&](const StepIn& u, int s, SegIn<8>& d) {
+    const std::uint32_t cell = u.rs + pix_off(u.rec);
+    const std::uint32_t j0 = static_cast<std::uint32_t>(min(seg_first<8>(s, l8), pix_num(u.rec) - 8));
+    const u32x2 c = ldb<u32x2>(Ct, cell + j0);
+    d.c[0] = c.x;
+    d.c[1] = c.y;
+    if constexpr (kAcc) {
+      const u32x4 v = ldb<u32x4>(St, (cell + j0) * 2U);
+      d.s[0] = v.x;
+      d.s[1] = v.y;
+      d.s[2] = v.z;
+      d.s[3] = v.w;
+    }
+  };
+  auto load_segs = [&](const StepIn& u, StepSegs& d) {
+    load_seg4(u, d.a);
+    load_seg8(u, 1, d.b);
   };
 
-  int prev_lo = 0, prev_num = 0, prev_ip = 0;
-  std::uint32_t prev_min = 0;
-  int flip = 0;
-  // Step i with its inputs u (this step), un (next) and un2 (receives step
-  // i + 2), v (this step) and vn (receives step i + 1).
-  auto step = [&](int i, StepIn& u, StepIn& un, StepIn& un2, StepLanes<kPre>& v, StepLanes<kPre>& vn) {
+  int prev_lo = 0, prev_num = 0, flip = 0;
+  std::uint32_t prev_ip = 0, prev_min = 0;
+  auto step = [&](int i, StepIn& u, StepIn& un, StepIn& un2, StepSegs& v, StepSegs& vn) {
     load_in(i + 2, un2);
-    load_lanes(i + 1, un, vn);
+    load_segs(un, vn);
     const bool act = grp_active && i < len;
-    const int ic = min(i, len - 1);
-    const int X = T.x0 + x0 + ic * DX;
-    const int lo = pix_lo(u.rec), num = pix_num(u.rec);
-    const std::uint64_t cell = T.cell_base + u.rs + pix_off(u.rec);
-    const bool lvalid = X >= u.rows.x && X <= u.rows.y;
-    const int rf = u.rows.z, rl = u.rows.w;
     const bool first = i == 0;
-    const std::uint32_t pen = min(prev_min + my_lut[abs(u.ip - prev_ip) >> 8], 0xFFFFU);
-    std::uint16_t* cur = buf + flip * rowlen;
-    const std::uint16_t* prv = buf + (flip ^ 1) * rowlen;
-    const int xr0 = X - J.dmin - lo;
+    const int lo = pix_lo(u.rec), num = pix_num(u.rec);
+    const std::uint32_t cell = u.rs + pix_off(u.rec);
+    const std::uint32_t pen =
+        min(prev_min + lut[wave][static_cast<std::uint32_t>(abs(static_cast<int>(u.ip) - static_cast<int>(prev_ip))) >> 8],
+            0xFFFFU);
+    const std::uint32_t PENx2 = pen * 0x10001U, MPx2 = prev_min * 0x10001U;
     const int delta = lo - prev_lo;
-    std::uint32_t m = 0xFFFFU;
-    auto label = [&](int j, bool on, std::uint32_t sv, std::uint64_t crv) {
-      const int xr = xr0 - j;
-      std::uint32_t c = xr >= rf && xr <= rl ? static_cast<std::uint32_t>(__popcll(u.cl ^ crv)) : invalid;
-      c = lvalid ? c : 0U;
-      const int k = j + delta;
-      const std::uint32_t same = static_cast<unsigned>(k) < static_cast<unsigned>(prev_num) ? prv[k] : 0xFFFFU;
-      const std::uint32_t a = static_cast<unsigned>(k - 1) < static_cast<unsigned>(prev_num) ? prv[k - 1] : 0xFFFFU;
-      const std::uint32_t b = static_cast<unsigned>(k + 1) < static_cast<unsigned>(prev_num) ? prv[k + 1] : 0xFFFFU;
-      const std::uint32_t nb = min(min(a, b) + P1, 0xFFFFU);
-      const std::uint32_t best = min(min(same, nb), pen);
-      const std::uint32_t L = first ? c : ((best - prev_min + c) & 0xFFFFU);
-      cur[j] = static_cast<std::uint16_t>(L);  // j < rowlen; unused past num
-      m = on ? min(m, L) : m;
-      const auto dst = on ? gS + cell + j : dummy;
-      *dst = static_cast<std::uint16_t>(kAcc ? sv + L : L);
+    const bool prev_wide = prev_num > kNarrow, cur_wide = num > kNarrow;
+    const int n_lds = min(prev_num, kNarrow);
+    const bool any_seg1 = __builtin_amdgcn_ballot_w64(act && num > 32) != 0;
+    const bool any_prev_wide = __builtin_amdgcn_ballot_w64(act && prev_wide) != 0;
+    const bool any_cur_wide = __builtin_amdgcn_ballot_w64(act && cur_wide) != 0;
+    std::uint32_t* cur = rows[wave][grp][flip];
+    const std::uint32_t* prv = rows[wave][grp][flip ^ 1];
+    __attribute__((address_space(1))) std::uint32_t* scur = srow + flip * swords;
+    const __attribute__((address_space(1))) std::uint32_t* sprv = srow + (flip ^ 1) * swords;
+    std::uint32_t m = 0xFFFFFFFFU;
+
+    // Byte position of predecessor label k - 1 for the window of a lane whose
+    // first label is j0; k is clamped into [-(P + 1), n + 1] so that the
+    // window stays inside the sentinels.
+    auto wpos = [&](int P, int j0, int n) {
+      return static_cast<std::uint32_t>(min(max(j0 + delta, -(P + 1)), n + 1) - 1 + kPad) * 2U;
     };
-    const int chunks = num >> 4;
+    // New path costs of P labels from their window words W (shift sh) and costs.
+    auto recur = [&](auto Pc, const std::uint32_t* W, std::uint32_t sh, const std::uint32_t* c, std::uint32_t* L) {
+      constexpr int P = decltype(Pc)::value;
+      std::uint32_t Q[P / 2 + 1];
 #pragma unroll
-    for (int q = 0; q < kPre; ++q) label(l16 + 16 * q, act && q < chunks, v.s[q], v.cr[q]);
-    if (chunks > kPre) {  // wide ranges: batches of 4 chunks, loads issued together
-      const auto crow = gcr + (T.y0 + y0 + ic * DY) * J.cr_stride;
-      for (int q = kPre; q < chunks; q += 4) {
-        std::uint32_t sv[4];
-        std::uint64_t cv[4];
+      for (int k = 0; k <= P / 2; ++k) Q[k] = __builtin_amdgcn_alignbyte(W[k + 1], W[k], sh);
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const int j = min(l16 + 16 * (q + b), num - 1);
-          sv[b] = kAcc ? static_cast<std::uint32_t>(gS[cell + j]) : 0U;
-          cv[b] = crow[min(max(xr0 - j, 0), pw1)];
+      for (int k = 0; k < P / 2; ++k) {
+        const std::uint32_t Cx2 = __builtin_amdgcn_perm(0U, c[k >> 1], (k & 1) ? 0x0c030c02U : 0x0c010c00U);
+        if (first) {
+          L[k] = Cx2;
+        } else {
+          const std::uint32_t p12 = __builtin_amdgcn_alignbyte(Q[k + 1], Q[k], 2U);
+          const u16x2 nb =
+              __builtin_elementwise_add_sat(__builtin_elementwise_min(as_pair(Q[k]), as_pair(Q[k + 1])), as_pair(P1x2));
+          const u16x2 best = __builtin_elementwise_min(__builtin_elementwise_min(as_pair(p12), nb), as_pair(PENx2));
+          L[k] = as_u32(best - as_pair(MPx2) + as_pair(Cx2));
         }
+      }
+    };
+    // LDS window of a lane.
+    auto lds_window = [&](auto Pc, int j0, std::uint32_t* W, std::uint32_t& sh) {
+      constexpr int P = decltype(Pc)::value;
+      sh = wpos(P, j0, n_lds);
+      const std::uint32_t* wp = prv + (sh >> 2);
 #pragma unroll
-        for (int b = 0; b < 4; ++b) label(l16 + 16 * (q + b), act && q + b < chunks, sv[b], cv[b]);
+      for (int k = 0; k < P / 2 + 2; ++k) W[k] = wp[k];
+    };
+    // Scratch window of a lane (in range for every lane; used where prev_wide).
+    auto scr_window = [&](auto Pc, int j0, std::uint32_t* W, std::uint32_t& sh) {
+      constexpr int P = decltype(Pc)::value;
+      sh = wpos(P, j0, prev_num);
+      const auto wp = sprv + (sh >> 2);
+#pragma unroll
+      for (int k = 0; k < P / 2 + 2; ++k) W[k] = wp[k];
+    };
+    // Stores of a lane's new path costs (next step's row) and S; running min.
+    auto emit = [&](auto Pc, int j0, const std::uint32_t* L, const std::uint32_t* sv) {
+      constexpr int P = decltype(Pc)::value;
+      const bool on = act && j0 < num;
+      const int w0 = (kPad + j0) >> 1;
+      if (on && !cur_wide) {
+#pragma unroll
+        for (int k = 0; k < P / 2; k += 2) *reinterpret_cast<u32x2*>(cur + w0 + k) = u32x2{L[k], L[k + 1]};
+      }
+      if (any_cur_wide) {
+        if (on && cur_wide) {
+#pragma unroll
+          for (int k = 0; k < P / 2; ++k) scur[w0 + k] = L[k];
+        }
+      }
+      std::uint32_t out[P / 2];
+      std::uint32_t mm = L[0];
+#pragma unroll
+      for (int k = 0; k < P / 2; ++k) {
+        out[k] = kAcc ? as_u32(as_pair(sv[k]) + as_pair(L[k])) : L[k];
+        if (k > 0) mm = as_u32(__builtin_elementwise_min(as_pair(mm), as_pair(L[k])));
+      }
+      if (on) {
+        const std::uint32_t off = (cell + static_cast<std::uint32_t>(j0)) * 2U;
+        if constexpr (P == 4) {
+          stb<u32x2>(St, off, u32x2{out[0], out[1]});
+        } else {
+          stb<u32x4>(St, off, u32x4{out[0], out[1], out[2], out[3]});
+        }
+      }
+      m = on ? as_u32(__builtin_elementwise_min(as_pair(m), as_pair(mm))) : m;
+    };
+    using P4 = std::integral_constant<int, 4>;
+    using P8 = std::integral_constant<int, 8>;
+    // One segment s >= 1 with its window from LDS or, where prev_wide, scratch.
+    auto seg8 = [&](int s, const SegIn<8>& d, bool scr) {
+      const int j0 = seg_first<8>(s, l8);
+      std::uint32_t W[6], sh, L[4];
+      lds_window(P8{}, j0, W, sh);
+      if (scr) {
+        std::uint32_t G[6], gsh;
+        scr_window(P8{}, j0, G, gsh);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) W[k] = prev_wide ? G[k] : W[k];
+        sh = prev_wide ? gsh : sh;
+      }
+      recur(P8{}, W, sh, d.c, L);
+      emit(P8{}, j0, L, d.s);
+    };
+    auto seg4 = [&](const SegIn<4>& d, bool scr) {
+      const int j0 = seg_first<4>(0, l8);
+      std::uint32_t W[4], sh, L[2];
+      lds_window(P4{}, j0, W, sh);
+      if (scr) {
+        std::uint32_t G[4], gsh;
+        scr_window(P4{}, j0, G, gsh);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) W[k] = prev_wide ? G[k] : W[k];
+        sh = prev_wide ? gsh : sh;
+      }
+      recur(P4{}, W, sh, d.c, L);
+      emit(P4{}, j0, L, d.s);
+    };
+    // Segments from 2 on (pixels wider than kNarrow), one segment ahead.
+    auto wide = [&](bool scr) {
+      const int segmax = wave_max(act ? segments(num) : 0);
+      SegIn<8> A, B;
+      load_seg8(u, 2, A);
+      for (int s = 2; s < segmax; s += 2) {
+        load_seg8(u, s + 1, B);
+        seg8(s, A, scr);
+        if (s + 1 >= segmax) break;
+        load_seg8(u, s + 2, A);
+        seg8(s + 1, B, scr);
+      }
+      // The loop leaves one segment load unconsumed: retire it here, so that
+      // no later use of its registers has to wait on the whole queue.
+      __builtin_amdgcn_s_waitcnt(kVmcnt0);
+    };
+    if (!any_prev_wide) {
+      seg4(v.a, false);
+      if (any_seg1) seg8(1, v.b, false);
+      if (any_cur_wide) wide(false);
+    } else {
+      seg4(v.a, true);
+      if (any_seg1) seg8(1, v.b, true);
+      if (any_cur_wide) wide(true);
+      __builtin_amdgcn_s_waitcnt(kVmcnt0);
+    }
+    // Sentinels after the labels.
+    if (act && l8 < kPad / 2) {
+      const int w = ((kPad + num) >> 1) + l8;
+      if (cur_wide) {
+        scur[w] = 0xFFFFFFFFU;
+      } else {
+        cur[w] = 0xFFFFFFFFU;
       }
     }
-    prev_min = row_min16(m);
+    prev_min = min8(min(m & 0xFFFFU, m >> 16));
     prev_lo = lo;
     prev_num = num;
     prev_ip = u.ip;
     flip ^= 1;
-    __builtin_amdgcn_wave_barrier();  // keeps this step's LDS writes before the next step's reads
+    __builtin_amdgcn_wave_barrier();  // this step's row writes before the next step's reads
   };
 
   StepIn ua, ub, uc;
-  StepLanes<kPre> va, vb;
+  StepSegs va, vb;
   load_in(0, ua);
   load_in(1, ub);
-  load_lanes(0, ua, va);
+  load_segs(ua, va);
   for (int i = 0; i < steps; i += 6) {
     step(i + 0, ua, ub, uc, va, vb);
     step(i + 1, ub, uc, ua, vb, va);
@@ -576,22 +817,15 @@ __global__ void k_pack_rows(const std::int32_t* __restrict__ base, const std::in
 
 template <int DX, int DY>
 void path_launch(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* gp, int ntiles,
-                 std::uint64_t groups, const std::uint64_t* rs, const std::uint64_t* pix, std::uint16_t* S,
-                 std::uint64_t total_cells, bool accumulate, int max_num, hipStream_t s) {
-  // Four scanlines per wave; 2 LDS rows of max_num labels per scanline plus
-  // one 256-entry P2 table per wave.
-  constexpr int kPre = 2;
-  const int rowlen = std::max(max_num, 16 * kPre) + 48;  // as in k_path
-  const auto lds_of = [&](int w) { return std::size_t(w) * (256 + 4 * 2 * std::size_t(rowlen)) * sizeof(std::uint16_t); };
-  const int waves = lds_of(4) <= 65536 ? 4 : 1;
-  const std::size_t lds = lds_of(waves);
-  const std::uint64_t blocks = (groups + waves - 1) / waves;
+                 std::uint64_t groups, const std::uint64_t* rs, const std::uint64_t* pix, const std::uint8_t* C,
+                 std::uint16_t* S, std::uint16_t* scratch, int scratch_row, bool accumulate, hipStream_t s) {
+  const std::uint64_t blocks = (groups + kWaves - 1) / kWaves;
   if (accumulate) {
-    k_path<DX, DY, kPre, true><<<dim3(static_cast<unsigned>(blocks)), 64 * waves, lds, s>>>(
-        jobs, tiles, gp, ntiles, groups, rs, pix, S, total_cells, max_num);
+    k_path<DX, DY, true><<<dim3(static_cast<unsigned>(blocks)), 64 * kWaves, 0, s>>>(
+        jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, scratch_row);
   } else {
-    k_path<DX, DY, kPre, false><<<dim3(static_cast<unsigned>(blocks)), 64 * waves, lds, s>>>(
-        jobs, tiles, gp, ntiles, groups, rs, pix, S, total_cells, max_num);
+    k_path<DX, DY, false><<<dim3(static_cast<unsigned>(blocks)), 64 * kWaves, 0, s>>>(
+        jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, scratch_row);
   }
 }
 
@@ -609,19 +843,35 @@ void launch_tile_cells(const TileDesc* tiles, int ntiles, const std::uint64_t* r
   k_tile_cells<<<dim3(ntiles), 256, 0, s>>>(tiles, row_cells, row_max, row_start, tile_cells, tile_max);
 }
 
+void launch_cost(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* row_prefix, int ntiles,
+                 std::uint64_t total_rows, const std::uint64_t* row_start, const std::uint64_t* pix, std::uint8_t* C,
+                 hipStream_t s) {
+  if (total_rows == 0) return;
+  k_cost<<<dim3(static_cast<unsigned>(total_rows)), 256, 0, s>>>(jobs, tiles, row_prefix, ntiles, row_start, pix, C);
+}
+
+int path_scratch_row(int max_num) { return kPad + (max_num + 15) / 16 * 16 + kPad; }
+
+std::size_t path_scratch_elems(std::uint64_t groups, int max_num) {
+  if (max_num <= kNarrow) return 0;
+  return std::size_t(groups) * kScanPerWave * 2 * std::size_t(path_scratch_row(max_num));
+}
+
 void launch_sgm_path(int path, const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* gp, int ntiles,
-                     std::uint64_t groups, const std::uint64_t* rs, const std::uint64_t* co, std::uint16_t* S,
-                     std::uint64_t total_cells, bool accumulate, int max_num, hipStream_t s) {
+                     std::uint64_t groups, const std::uint64_t* rs, const std::uint64_t* pix, const std::uint8_t* C,
+                     std::uint16_t* S, std::uint16_t* scratch, int max_num, bool accumulate, hipStream_t s) {
   if (groups == 0) return;
+  const int srow = path_scratch_row(max_num);
+  if (max_num <= kNarrow) scratch = nullptr;
   switch (path) {
-    case 0: path_launch<1, 0>(jobs, tiles, gp, ntiles, groups, rs, co, S, total_cells, accumulate, max_num, s); break;
-    case 1: path_launch<-1, 0>(jobs, tiles, gp, ntiles, groups, rs, co, S, total_cells, accumulate, max_num, s); break;
-    case 2: path_launch<0, 1>(jobs, tiles, gp, ntiles, groups, rs, co, S, total_cells, accumulate, max_num, s); break;
-    case 3: path_launch<0, -1>(jobs, tiles, gp, ntiles, groups, rs, co, S, total_cells, accumulate, max_num, s); break;
-    case 4: path_launch<1, 1>(jobs, tiles, gp, ntiles, groups, rs, co, S, total_cells, accumulate, max_num, s); break;
-    case 5: path_launch<-1, -1>(jobs, tiles, gp, ntiles, groups, rs, co, S, total_cells, accumulate, max_num, s); break;
-    case 6: path_launch<-1, 1>(jobs, tiles, gp, ntiles, groups, rs, co, S, total_cells, accumulate, max_num, s); break;
-    default: path_launch<1, -1>(jobs, tiles, gp, ntiles, groups, rs, co, S, total_cells, accumulate, max_num, s); break;
+    case 0: path_launch<1, 0>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, accumulate, s); break;
+    case 1: path_launch<-1, 0>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, accumulate, s); break;
+    case 2: path_launch<0, 1>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, accumulate, s); break;
+    case 3: path_launch<0, -1>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, accumulate, s); break;
+    case 4: path_launch<1, 1>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, accumulate, s); break;
+    case 5: path_launch<-1, -1>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, accumulate, s); break;
+    case 6: path_launch<-1, 1>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, accumulate, s); break;
+    default: path_launch<1, -1>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, accumulate, s); break;
   }
 }
 

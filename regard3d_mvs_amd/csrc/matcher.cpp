@@ -455,7 +455,8 @@ Aggregation aggregate_jobs(Context& c, std::vector<Job>& jobs, const r3dg_option
   for (int p = 0; p < 8; ++p) {
     std::uint64_t* g = gp.data() + std::size_t(p) * (std::size_t(nt) + 1);
     for (int t = 0; t < nt; ++t)
-      g[t + 1] = g[t] + std::uint64_t((path_scanlines(p, td[std::size_t(t)].w, td[std::size_t(t)].h) + 3) / 4);
+      g[t + 1] = g[t] + std::uint64_t((path_scanlines(p, td[std::size_t(t)].w, td[std::size_t(t)].h) +
+                                       kScanPerWave - 1) / kScanPerWave);
   }
   DBuf<std::uint64_t> dgp;
   upload(c, dgp, gp);
@@ -463,18 +464,35 @@ Aggregation aggregate_jobs(Context& c, std::vector<Job>& jobs, const r3dg_option
   DBuf<std::uint16_t>& S = A.S;
   layout_timer.reset();
 
+  // Scratch rows for pixels wider than the kernel's LDS rows.
+  std::uint64_t max_groups = 0;
+  for (int p = 0; p < 8; ++p) max_groups = std::max(max_groups, gp[std::size_t(p) * (std::size_t(nt) + 1) + nt]);
+  DBuf<std::uint16_t> scratch;
+  if (const std::size_t n = path_scratch_elems(max_groups, max_num); n > 0) scratch = DBuf<std::uint16_t>(c.arena, n);
+
+  // Matching costs of every cell (timed with the other cost preparation).
+  DBuf<std::uint8_t> C(c.arena, total_cells + kSPad);
+  {
+    const StageTimer timer(c, R3DG_STAGE_PREPARE);
+    launch_cost(djobs.get(), dtiles.get(), drow_prefix.get(), nt, total_rows, row_start.get(), pix.get(), C.get(),
+                c.stream);
+    check_launch();
+  }
+
   // Aggregation: 8 path launches over all tiles.
   {
     const StageTimer timer(c, R3DG_STAGE_AGGREGATE);
     for (int p = 0; p < 8; ++p) {
       const std::size_t o8 = std::size_t(p) * (std::size_t(nt) + 1);
       launch_sgm_path(p, djobs.get(), dtiles.get(), dgp.get() + o8, nt, gp[o8 + std::size_t(nt)], row_start.get(),
-                      pix.get(), S.get(), total_cells, p > 0, max_num, c.stream);
+                      pix.get(), C.get(), S.get(), scratch.get(), max_num, p > 0, c.stream);
       check_launch();
     }
   }
   c.agg_launches += 8;
-  c.agg_bytes += 30 * total_cells + 176 * total_px;
+  // Algorithmic bytes of the 8 launches: per cell C read 8 x 1, S written
+  // 8 x 2 and read 7 x 2; per tile pixel and path its record (8) and intensity (2).
+  c.agg_bytes += 38 * total_cells + 80 * total_px;
 
   return A;
 }
