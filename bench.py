@@ -69,6 +69,125 @@ def cpu_baseline(pair, options, rows, threads):
                       f"{r.levels + 1} levels, {r.cells} cells) in {dt:.2f} s, scaled to whole pairs"}
 
 
+def _same(a, b):
+    return a.shape == b.shape and bool(np.array_equal(a.view(np.uint32), b.view(np.uint32)))
+
+
+def bench_batch(args) -> int:
+    """The scene of views x 4 partners (1.5-24 MP, spans 64..512) matched as 160
+    plain r3dg_match calls and as one r3dg_match_batch; host images in, host
+    maps out for both."""
+    from regard3d_mvs_amd import StereoOptions, workloads
+    from regard3d_mvs_amd.native import Matcher
+
+    t = time.perf_counter()
+    src = workloads.source_pair()
+    reqs = workloads.scene(src, views=args.views)
+    log(f"bench: source {workloads.SOURCE_W}x{workloads.SOURCE_H} and {len(reqs)} pairs in "
+        f"{time.perf_counter() - t:.1f} s; {sum(r.left.size for r in reqs) / len(reqs) / 1e6:.1f} MP mean")
+    o = StereoOptions()
+    m = Matcher(0)
+    m.match(reqs[0].left, reqs[0].right, None, None, reqs[0].dmin, reqs[0].nd, o)  # warm-up
+    t0 = time.perf_counter()
+    plain = [m.match(r.left, r.right, None, None, r.dmin, r.nd, o) for r in reqs]
+    t_plain = time.perf_counter() - t0
+    log(f"bench: plain {len(reqs)} pairs in {t_plain:.2f} s")
+    batch = [dict(left=r.left, right=r.right, dmin=r.dmin, nd=r.nd) for r in reqs]
+    t0 = time.perf_counter()
+    got = m.match_batch(batch, o)
+    t_batch = time.perf_counter() - t0
+    log(f"bench: batched {len(reqs)} pairs in {t_batch:.2f} s")
+    exact = all(_same(g.disparity, p.disparity) and _same(g.confidence, p.confidence) for g, p in zip(got, plain))
+    cells = sum(p.cells for p in plain)
+    out = {
+        "metric": "pairs/s, batched scene vs plain per-pair calls", "unit": "pairs/s", "n_gpus": 1,
+        "value": len(reqs) / t_batch, "plain_pairs_per_s": len(reqs) / t_plain,
+        "speedup_vs_plain": t_plain / t_batch, "higher_is_better": True, "data": "synthetic", "dtype": "u16",
+        "bit_exact_vs_plain": exact, "pairs": len(reqs), "cells": cells,
+        "gcell_path_per_s": cells * 8 / t_batch / 1e9, "plain_gcell_path_per_s": cells * 8 / t_plain / 1e9,
+        "config": {"workload": f"{args.views} views x 4 partners, 1.5-24 MP (3:2 or 4:3), right widths +/-15%, "
+                               "spans log-uniform 64..512, crops of a 6400x4400 span-512 synthetic pair",
+                   "inputs": "host uint16, outputs host float"},
+    }
+    print(json.dumps(out), flush=True)
+    m.close()
+    return 0
+
+
+def bench_serve(args) -> int:
+    """Poisson request streams through the serving loop (left views cached,
+    right images uploaded per request), against one-at-a-time r3dg_match calls
+    of the same requests in arrival order."""
+    import threading
+
+    from regard3d_mvs_amd import StereoOptions, workloads
+    from regard3d_mvs_amd.native import Matcher
+
+    src = workloads.source_pair()
+    reqs = workloads.stream(src, args.requests)
+    log(f"bench: {len(reqs)} requests, {sum(r.left.size for r in reqs) / len(reqs) / 1e6:.2f} MP mean, "
+        f"{len(set(r.view for r in reqs))} views")
+    o = StereoOptions()
+    m = Matcher(0)
+    m.match(reqs[0].left, reqs[0].right, None, None, reqs[0].dmin, reqs[0].nd, o)  # warm-up
+    service, plain = [], []
+    for r in reqs:
+        s0 = time.perf_counter()
+        plain.append(m.match(r.left, r.right, None, None, r.dmin, r.nd, o).disparity)
+        service.append(time.perf_counter() - s0)
+    log(f"bench: plain service {sum(service):.2f} s ({len(reqs) / sum(service):.2f} pairs/s)")
+    results = []
+    for rate in [float(x) for x in args.rates.split(",")]:
+        arrivals = workloads.poisson_arrivals(len(reqs), rate)
+        m.serve_start(o, hbm_budget=int(args.hbm_budget * 1e9))
+        tickets = [None] * len(reqs)
+        put = set()
+        t0 = time.perf_counter()
+
+        def client():
+            for i, (a, r) in enumerate(zip(arrivals, reqs)):
+                d = t0 + a - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                if r.view not in put:
+                    m.view_put(r.view, r.left)
+                    put.add(r.view)
+                tickets[i] = m.submit(r.left.shape, r.dmin, r.nd, right=r.right, left_view=r.view)
+
+        th = threading.Thread(target=client)
+        th.start()
+        lat, exact, done = [], True, 0
+        for i in range(len(reqs)):
+            while tickets[i] is None:
+                time.sleep(0.0005)
+            g = m.wait(tickets[i])
+            lat.append(g.seconds)
+            exact = exact and _same(g.disparity, plain[i])
+            done = time.perf_counter()
+        th.join()
+        m.serve_stop()
+        for v in put:
+            m.view_evict(v)
+        span = done - t0
+        fl = workloads.fifo_latencies(arrivals, service)
+        res = {"rate": rate, "pairs_per_s": len(reqs) / span,
+               "latency_ms": {q: 1e3 * percentile(lat, int(q[1:])) for q in ("p50", "p95", "p99")},
+               "plain_fifo_latency_ms": {q: 1e3 * percentile(fl, int(q[1:])) for q in ("p50", "p95", "p99")},
+               "plain_pairs_per_s": len(reqs) / float(np.max(arrivals + fl)),
+               "bit_exact_vs_plain": exact}
+        log(f"bench: rate {rate}: {json.dumps(res)}")
+        results.append(res)
+    out = {"metric": "served pairs/s and p50/p95/p99 latency vs plain", "unit": "pairs/s", "n_gpus": 1,
+           "value": results[-1]["pairs_per_s"], "plain_capacity_pairs_per_s": len(reqs) / sum(service),
+           "higher_is_better": True, "data": "synthetic", "dtype": "u16", "rates": results,
+           "config": {"workload": f"{len(reqs)} requests per rate, Poisson arrivals, 0.3-24 MP log-uniform, "
+                                  "spans log-uniform 32..512, views reused by 1-4 partners, crops of a "
+                                  "6400x4400 span-512 synthetic pair", "hbm_budget_gb": args.hbm_budget}}
+    print(json.dumps(out), flush=True)
+    m.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,11 +196,22 @@ def main() -> int:
     ap.add_argument("--config", type=int, default=2, choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-rows", type=int, default=2016, help="rows of the CPU-baseline band (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--workload", default="pair", choices=["pair", "batch", "serve"],
+                    help="pair: one pair per call (default); batch: the 160-pair scene, batched vs plain; "
+                         "serve: a Poisson request stream through the serving loop vs plain")
+    ap.add_argument("--views", type=int, default=40, help="batch: views (4 partners each)")
+    ap.add_argument("--requests", type=int, default=300, help="serve: requests per rate")
+    ap.add_argument("--rates", default="5,20,50", help="serve: arrival rates (pairs/s)")
+    ap.add_argument("--hbm-budget", type=float, default=200.0, help="serve: admission budget (GB)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(pairs are independent, see DESIGN.md for how they would shard)", file=sys.stderr)
         return 2
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     import torch
 

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define R3DG_ABI_VERSION 1
+#define R3DG_ABI_VERSION 2
 
 typedef enum r3dg_status {
   R3DG_OK = 0,
@@ -38,7 +38,8 @@ typedef enum r3dg_status {
   R3DG_ERROR_CANCELLED = 4,        /* maps to r3d::Cancelled (core/cancel.hpp:10-13) */
   R3DG_ERROR_BUDGET = 5,           /* request larger than the context's HBM budget */
   R3DG_ERROR_NOT_FOUND = 6,        /* unknown ticket or view id */
-  R3DG_ERROR_INTERNAL = 7
+  R3DG_ERROR_INTERNAL = 7,
+  R3DG_PENDING = 8                 /* a submitted pair that has not finished yet */
 } r3dg_status;
 
 typedef enum r3dg_memory { R3DG_HOST = 0, R3DG_DEVICE = 1 } r3dg_memory;
@@ -107,7 +108,7 @@ typedef struct r3dg_match_info {
   uint64_t tile_pixels;    /* sum of tile pixels over levels and directions */
   double seconds;          /* wall time of the call */
   uint64_t aggregate_launches; /* path-kernel launches */
-  uint64_t aggregate_bytes;    /* algorithmic bytes of the path kernels: sum of 30 C + 176 P */
+  uint64_t aggregate_bytes;    /* algorithmic bytes of the path kernels: sum of 38 C + 80 P */
   double stage_seconds[R3DG_STAGES]; /* device time per stage (0 unless profiling) */
 } r3dg_match_info;
 
@@ -119,6 +120,21 @@ typedef struct r3dg_match_debug {
   r3dg_image labels;
   r3dg_image lr_mask;
 } r3dg_match_debug;
+
+/* One pair of r3dg_match_batch: inputs as for r3dg_match, outputs written by
+ * the call (status: R3DG_OK or this pair's error; info.seconds and
+ * info.stage_seconds are those of the whole batch). */
+typedef struct r3dg_pair {
+  r3dg_image left, right;             /* uint16 */
+  r3dg_image left_valid, right_valid; /* uint8, empty views: all valid */
+  int32_t disparity_min;
+  int32_t num_disparities;
+  r3dg_image disparity;               /* float, left size */
+  r3dg_image confidence;              /* float, left size, may be empty */
+  r3dg_match_info info;
+  int32_t status;
+  int32_t reserved;
+} r3dg_pair;
 
 typedef struct r3dg_ctx r3dg_ctx;
 
@@ -177,6 +193,46 @@ int32_t r3dg_match(r3dg_ctx* ctx, r3dg_image left, r3dg_image right, r3dg_image 
                    r3dg_image right_valid, int32_t disparity_min, int32_t num_disparities,
                    const r3dg_options* options, r3dg_image disparity, r3dg_image confidence,
                    const r3dg_match_debug* debug, r3dg_match_info* info);
+
+/* match_stereo for several pairs in one call; no counterpart in the reference,
+ * whose callers loop over pairs (mvs/multi_stereo.cpp:264-293 calls
+ * match_stereo once per partner).  The levels of all pairs run in lockstep,
+ * finest levels together, and the jobs of a level share the cost, path and
+ * selection launches; pairs are grouped so that a group fits the context's
+ * memory budget.  Every pair's outputs are bit-identical to r3dg_match on it
+ * alone.  Pairs that fail validation get their status and are skipped; the
+ * call returns the first pair error, or R3DG_OK. */
+int32_t r3dg_match_batch(r3dg_ctx* ctx, r3dg_pair* pairs, int32_t count, const r3dg_options* options);
+
+/* ---- serving (no counterpart in the reference; for callers that match
+ * pairs as they arrive) ---------------------------------------------------
+ * r3dg_server_start runs a request loop on its own thread.  Submitted pairs
+ * wait until their estimated device footprint fits hbm_budget (0: 80% of the
+ * context budget) next to the pairs in flight; every step of the loop then
+ * matches the next level of every admitted pair with shared launches, so new
+ * pairs join while others are on the GPU (continuous batching).  A finished
+ * pair's outputs are written and its ticket completes; pair->info.seconds is
+ * the time from submission to completion.  A pair and the images it points to
+ * must stay valid until its ticket completes.  Each pair's outputs equal
+ * those of r3dg_match on it alone.  Direct calls on the context interleave
+ * with the loop's steps. */
+int32_t r3dg_server_start(r3dg_ctx* ctx, const r3dg_options* options, uint64_t hbm_budget);
+/* Finishes the submitted pairs, then stops the loop. */
+int32_t r3dg_server_stop(r3dg_ctx* ctx);
+/* left_view / right_view: ids given to r3dg_view_put, used in place of the
+ * pair's left + left_valid (right + right_valid) images when non-zero. */
+int32_t r3dg_submit(r3dg_ctx* ctx, r3dg_pair* pair, uint64_t left_view, uint64_t right_view, uint64_t* ticket);
+/* R3DG_PENDING while the pair runs, else its status (the ticket is released). */
+int32_t r3dg_poll(r3dg_ctx* ctx, uint64_t ticket);
+/* Waits up to timeout_seconds (< 0: no limit); returns as r3dg_poll. */
+int32_t r3dg_wait(r3dg_ctx* ctx, uint64_t ticket, double timeout_seconds);
+/* Keeps an image (uint16) and its mask (uint8, may be empty) on the device
+ * under a non-zero id, with its pyramid, Census and mirrored copies built on
+ * first use and reused by every pair that names the id.  Replaces an existing
+ * view of that id. */
+int32_t r3dg_view_put(r3dg_ctx* ctx, uint64_t view, r3dg_image image, r3dg_image valid);
+/* Drops a view; pairs in flight that use it keep it until they finish. */
+int32_t r3dg_view_evict(r3dg_ctx* ctx, uint64_t view);
 
 /* remove_speckles (stereo_matcher.hpp:89), in place on a float image. */
 int32_t r3dg_remove_speckles(r3dg_ctx* ctx, r3dg_image disparity, int32_t min_size, float max_diff);

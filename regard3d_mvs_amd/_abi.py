@@ -21,6 +21,7 @@ class Status(enum.IntEnum):
     BUDGET = 5
     NOT_FOUND = 6
     INTERNAL = 7
+    PENDING = 8
 
 
 class SubpixelMethod(enum.IntEnum):
@@ -58,6 +59,14 @@ class CMatchInfo(ctypes.Structure):
 
 class CMatchDebug(ctypes.Structure):
     _fields_ = [("labels", CImage), ("lr_mask", CImage)]
+
+
+class CPair(ctypes.Structure):
+    """r3dg_pair: one pair of r3dg_match_batch."""
+    _fields_ = [("left", CImage), ("right", CImage), ("left_valid", CImage), ("right_valid", CImage),
+                ("disparity_min", ctypes.c_int32), ("num_disparities", ctypes.c_int32), ("disparity", CImage),
+                ("confidence", CImage), ("info", CMatchInfo), ("status", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 @dataclasses.dataclass

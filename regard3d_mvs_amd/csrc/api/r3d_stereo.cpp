@@ -12,6 +12,7 @@
 
 #include "r3d/core/pyramid.hpp"
 #include "r3d/stereo/census.hpp"
+#include "r3d/stereo/stereo_batch.hpp"
 #include "r3d/stereo/stereo_matcher.hpp"
 #include "r3dg.h"
 
@@ -115,6 +116,35 @@ DisparityMap match_stereo(ImageView<const std::uint16_t> left, ImageView<const s
   out.num_disparities = info.num_disparities;
   out.levels = info.levels;
   out.cells = info.cells;
+  return out;
+}
+
+std::vector<DisparityMap> match_stereo_batch(const std::vector<StereoPair>& pairs, const StereoOptions& options) {
+  std::vector<DisparityMap> out(pairs.size());
+  std::vector<r3dg_pair> desc_pairs(pairs.size());
+  for (std::size_t i = 0; i < pairs.size(); ++i) {
+    const StereoPair& p = pairs[i];
+    out[i].disparity = ImageF32::uninitialized(p.left.width(), p.left.height(), 1);
+    out[i].confidence = ImageF32::uninitialized(p.left.width(), p.left.height(), 1);
+    r3dg_pair& q = desc_pairs[i];
+    q = r3dg_pair{};
+    q.left = desc(p.left);
+    q.right = desc(p.right);
+    q.left_valid = desc(p.left_valid);
+    q.right_valid = desc(p.right_valid);
+    q.disparity_min = p.disparity_min;
+    q.num_disparities = p.num_disparities;
+    q.disparity = desc(out[i].disparity);
+    q.confidence = desc(out[i].confidence);
+  }
+  const r3dg_options o = options_of(options);
+  check(r3dg_match_batch(context(), desc_pairs.data(), static_cast<int32_t>(desc_pairs.size()), &o));
+  for (std::size_t i = 0; i < pairs.size(); ++i) {
+    out[i].disparity_min = desc_pairs[i].info.disparity_min;
+    out[i].num_disparities = desc_pairs[i].info.num_disparities;
+    out[i].levels = desc_pairs[i].info.levels;
+    out[i].cells = desc_pairs[i].info.cells;
+  }
   return out;
 }
 

@@ -70,8 +70,8 @@ void launch_cost(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t
                  std::uint64_t total_rows, const std::uint64_t* row_start, const std::uint64_t* pix, std::uint8_t* C,
                  hipStream_t s);
 // One SGM path over every tile (path index 0..7, see kernels_sgm.hip).
-// `scratch` holds path_scratch_elems(groups, max_num) uint16 (rows of pixels
-// wider than the LDS rows); it may be null when that is 0.
+// `scratch` holds path_scratch_elems(groups, max_num) uint16: per scanline
+// two rows for the path costs of pixels wider than the LDS rows.
 std::size_t path_scratch_elems(std::uint64_t groups, int max_num);
 int path_scratch_row(int max_num);
 void launch_sgm_path(int path, const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* group_prefix, int ntiles,

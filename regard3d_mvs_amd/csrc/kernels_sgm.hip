@@ -20,6 +20,7 @@
 
 #include <cstdint>
 #include <type_traits>
+#include <utility>
 
 #include "kernels.hpp"
 
@@ -276,6 +277,7 @@ __device__ __forceinline__ void stb(void* base, std::uint32_t off, T v) {
 // operation and nothing else: vmcnt 0, expcnt 7, lgkmcnt 15.
 constexpr int kVmcnt0 = 0x0F70;
 constexpr int kWaves = 4;                     // waves per block
+constexpr int kDist = 1;                      // steps between a segment prefetch and its use
 constexpr int kNarrow = 96;                   // labels whose path costs stay in LDS
 constexpr int kPad = 12;                      // sentinel slots before label 0 and after the last label
 constexpr int kRow = kPad + kNarrow + kPad;   // uint16 slots of an LDS row
@@ -319,8 +321,8 @@ struct StepSegs {
 // most kNarrow labels and in a global scratch row for wider ones; a lane reads
 // its window of P + 2 values as aligned words (the window clamped into the
 // sentinels) and shifts them into place.  Segments 0 and 1 use data prefetched
-// a step ahead (pixel records two steps ahead); segments from 2 on (pixels
-// wider than kNarrow) are loaded in the step, one segment ahead.
+// kDist steps ahead (pixel records one step earlier still); segments from 2 on
+// (pixels of more than 96 labels) are loaded in the step, one segment ahead.
 template <int DX, int DY, bool kAcc>
 __global__ __launch_bounds__(64 * kWaves) void k_path(
     const JobDesc* __restrict__ jobs, const TileDesc* __restrict__ tiles, const std::uint64_t* __restrict__ gprefix,
@@ -416,9 +418,13 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
 
   int prev_lo = 0, prev_num = 0, flip = 0;
   std::uint32_t prev_ip = 0, prev_min = 0;
-  auto step = [&](int i, StepIn& u, StepIn& un, StepIn& un2, StepSegs& v, StepSegs& vn) {
-    load_in(i + 2, un2);
-    load_segs(un, vn);
+  // Step i: u, v are its pixel values and segments; it loads the pixel values
+  // of step i + kDist + 1 into u_load and the segments of step i + kDist
+  // (pixel values u_seg) into v_load.
+  auto step = [&](int i, const StepIn& u, const StepSegs& v, StepIn& u_load, const StepIn& u_seg,
+                  StepSegs& v_load) __attribute__((always_inline)) {
+    load_in(i + kDist + 1, u_load);
+    load_segs(u_seg, v_load);
     const bool act = grp_active && i < len;
     const bool first = i == 0;
     const int lo = pix_lo(u.rec), num = pix_num(u.rec);
@@ -431,6 +437,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
     const bool prev_wide = prev_num > kNarrow, cur_wide = num > kNarrow;
     const int n_lds = min(prev_num, kNarrow);
     const bool any_seg1 = __builtin_amdgcn_ballot_w64(act && num > 32) != 0;
+    const bool any_seg2 = __builtin_amdgcn_ballot_w64(act && num > 96) != 0;
     const bool any_prev_wide = __builtin_amdgcn_ballot_w64(act && prev_wide) != 0;
     const bool any_cur_wide = __builtin_amdgcn_ballot_w64(act && cur_wide) != 0;
     std::uint32_t* cur = rows[wave][grp][flip];
@@ -515,61 +522,74 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
     };
     using P4 = std::integral_constant<int, 4>;
     using P8 = std::integral_constant<int, 8>;
-    // One segment s >= 1 with its window from LDS or, where prev_wide, scratch.
-    auto seg8 = [&](int s, const SegIn<8>& d, bool scr) {
-      const int j0 = seg_first<8>(s, l8);
-      std::uint32_t W[6], sh, L[4];
-      lds_window(P8{}, j0, W, sh);
-      if (scr) {
-        std::uint32_t G[6], gsh;
-        scr_window(P8{}, j0, G, gsh);
+    // A lane's window: from LDS, or from its scratch row where prev_wide.  The
+    // scratch loads are retired inside their block, so that the shared code
+    // after it never waits on them.
+    auto window = [&](auto Pc, int j0, std::uint32_t* W, std::uint32_t& sh) {
+      constexpr int P = decltype(Pc)::value;
+      lds_window(Pc, j0, W, sh);
+      if (any_prev_wide) {
+        std::uint32_t G[P / 2 + 2], gsh;
+        scr_window(Pc, j0, G, gsh);
+        __builtin_amdgcn_s_waitcnt(kVmcnt0);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) W[k] = prev_wide ? G[k] : W[k];
+        for (int k = 0; k < P / 2 + 2; ++k) W[k] = prev_wide ? G[k] : W[k];
         sh = prev_wide ? gsh : sh;
       }
+    };
+    auto seg8 = [&](int s, const SegIn<8>& d) {
+      const int j0 = seg_first<8>(s, l8);
+      std::uint32_t W[6], sh, L[4];
+      window(P8{}, j0, W, sh);
       recur(P8{}, W, sh, d.c, L);
       emit(P8{}, j0, L, d.s);
     };
-    auto seg4 = [&](const SegIn<4>& d, bool scr) {
+    auto seg4 = [&](const SegIn<4>& d) {
       const int j0 = seg_first<4>(0, l8);
       std::uint32_t W[4], sh, L[2];
-      lds_window(P4{}, j0, W, sh);
-      if (scr) {
-        std::uint32_t G[4], gsh;
-        scr_window(P4{}, j0, G, gsh);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) W[k] = prev_wide ? G[k] : W[k];
-        sh = prev_wide ? gsh : sh;
-      }
+      window(P4{}, j0, W, sh);
       recur(P4{}, W, sh, d.c, L);
       emit(P4{}, j0, L, d.s);
     };
-    // Segments from 2 on (pixels wider than kNarrow), one segment ahead.
-    auto wide = [&](bool scr) {
+    // Segments from 2 on (pixels of more than 96 labels): costs, S and the
+    // scratch window of every lane (used where prev_wide) loaded one segment
+    // ahead.  No load in the loop is conditional and the loop has one exit,
+    // so the waits inside it only cover the segment being computed.
+    auto wide = [&]() {
       const int segmax = wave_max(act ? segments(num) : 0);
-      SegIn<8> A, B;
-      load_seg8(u, 2, A);
+      struct Ahead {
+        SegIn<8> d;
+        std::uint32_t G[6];
+      } A, B;
+      auto load = [&](int s, Ahead& x) {
+        load_seg8(u, s, x.d);
+        std::uint32_t gsh;
+        scr_window(P8{}, seg_first<8>(s, l8), x.G, gsh);
+      };
+      auto run = [&](int s, const Ahead& x) {
+        const int j0 = seg_first<8>(s, l8);
+        std::uint32_t W[6], sh, L[4];
+        lds_window(P8{}, j0, W, sh);
+        const std::uint32_t gsh = wpos(8, j0, prev_num);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) W[k] = prev_wide ? x.G[k] : W[k];
+        sh = prev_wide ? gsh : sh;
+        recur(P8{}, W, sh, x.d.c, L);
+        emit(P8{}, j0, L, x.d.s);
+      };
+      load(2, A);
       for (int s = 2; s < segmax; s += 2) {
-        load_seg8(u, s + 1, B);
-        seg8(s, A, scr);
-        if (s + 1 >= segmax) break;
-        load_seg8(u, s + 2, A);
-        seg8(s + 1, B, scr);
+        load(s + 1, B);
+        run(s, A);
+        load(s + 2, A);
+        if (s + 1 < segmax) run(s + 1, B);
       }
-      // The loop leaves one segment load unconsumed: retire it here, so that
-      // no later use of its registers has to wait on the whole queue.
+      // Retire the loads of the segments past the end.
       __builtin_amdgcn_s_waitcnt(kVmcnt0);
     };
-    if (!any_prev_wide) {
-      seg4(v.a, false);
-      if (any_seg1) seg8(1, v.b, false);
-      if (any_cur_wide) wide(false);
-    } else {
-      seg4(v.a, true);
-      if (any_seg1) seg8(1, v.b, true);
-      if (any_cur_wide) wide(true);
-      __builtin_amdgcn_s_waitcnt(kVmcnt0);
-    }
+    seg4(v.a);
+    if (any_seg1) seg8(1, v.b);
+    if (any_seg2) wide();
     // Sentinels after the labels.
     if (act && l8 < kPad / 2) {
       const int w = ((kPad + num) >> 1) + l8;
@@ -587,18 +607,21 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
     __builtin_amdgcn_wave_barrier();  // this step's row writes before the next step's reads
   };
 
-  StepIn ua, ub, uc;
-  StepSegs va, vb;
-  load_in(0, ua);
-  load_in(1, ub);
-  load_segs(ua, va);
-  for (int i = 0; i < steps; i += 6) {
-    step(i + 0, ua, ub, uc, va, vb);
-    step(i + 1, ub, uc, ua, vb, va);
-    step(i + 2, uc, ua, ub, va, vb);
-    step(i + 3, ua, ub, uc, vb, va);
-    step(i + 4, ub, uc, ua, va, vb);
-    step(i + 5, uc, ua, ub, vb, va);
+  // Register rotation: kDist + 2 sets of pixel values, kDist + 1 of segments,
+  // the loop unrolled over a full period so every index is a constant.
+  constexpr int NU = kDist + 2, NV = kDist + 1;
+  StepIn U[NU];
+  StepSegs V[NV];
+  for (int j = 0; j <= kDist; ++j) load_in(j, U[j]);
+  for (int j = 0; j < kDist; ++j) load_segs(U[j], V[j]);
+  auto body = [&](auto Kc, int i) __attribute__((always_inline)) {
+    constexpr int K = decltype(Kc)::value;
+    step(i, U[K % NU], V[K % NV], U[(K + kDist + 1) % NU], U[(K + kDist) % NU], V[(K + kDist) % NV]);
+  };
+  for (int i = 0; i < steps; i += NU * NV) {
+    [&]<int... K>(std::integer_sequence<int, K...>) __attribute__((always_inline)) {
+      (body(std::integral_constant<int, K>{}, i + K), ...);
+    }(std::make_integer_sequence<int, NU * NV>{});
   }
 }
 
@@ -853,7 +876,6 @@ void launch_cost(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t
 int path_scratch_row(int max_num) { return kPad + (max_num + 15) / 16 * 16 + kPad; }
 
 std::size_t path_scratch_elems(std::uint64_t groups, int max_num) {
-  if (max_num <= kNarrow) return 0;
   return std::size_t(groups) * kScanPerWave * 2 * std::size_t(path_scratch_row(max_num));
 }
 
@@ -862,7 +884,6 @@ void launch_sgm_path(int path, const JobDesc* jobs, const TileDesc* tiles, const
                      std::uint16_t* S, std::uint16_t* scratch, int max_num, bool accumulate, hipStream_t s) {
   if (groups == 0) return;
   const int srow = path_scratch_row(max_num);
-  if (max_num <= kNarrow) scratch = nullptr;
   switch (path) {
     case 0: path_launch<1, 0>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, accumulate, s); break;
     case 1: path_launch<-1, 0>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, accumulate, s); break;

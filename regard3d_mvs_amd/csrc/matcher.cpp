@@ -11,11 +11,16 @@
 #include <algorithm>
 #include <array>
 #include <chrono>
+#include <condition_variable>
 #include <cmath>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <memory>
 #include <optional>
 #include <string>
+#include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "kernels.hpp"
@@ -234,6 +239,61 @@ void post_filter(Context& c, DImg<float>& d, const r3dg_options& o, int speckle)
 
 // ------------------------------------------------------------------ tiles
 // One matching direction of one level (MatchContext, stereo_matcher.cpp:113-163).
+// A device image with its pyramid, Census and mirrored copies per level,
+// each computed on first use.  Single matches build one per input image; the
+// serving loop keeps them across the pairs that share a view.  deque and map
+// keep element addresses stable, so jobs may point into them.
+struct View {
+  DImg<std::uint16_t> img;
+  DImg<std::uint8_t> mask;
+  std::deque<DImg<std::uint16_t>> pyr;  // level l at pyr[l - 1]
+  std::deque<DImg<std::uint8_t>> mpyr;
+  std::map<int, DImg<std::uint64_t>> cen, cen_m;
+  std::map<int, DImg<std::uint16_t>> mir;
+  std::map<int, DImg<std::uint8_t>> mir_mask;
+
+  const DImg<std::uint16_t>& image(Context& c, int l) {
+    while (static_cast<int>(pyr.size()) < l) pyr.push_back(pyramid_u16(c, pyr.empty() ? img : pyr.back()));
+    return l == 0 ? img : pyr[std::size_t(l - 1)];
+  }
+  const DImg<std::uint8_t>& valid(Context& c, int l) {
+    if (mask.empty()) return mask;
+    while (static_cast<int>(mpyr.size()) < l) mpyr.push_back(pyramid_u8(c, mpyr.empty() ? mask : mpyr.back(), true));
+    return l == 0 ? mask : mpyr[std::size_t(l - 1)];
+  }
+  const DImg<std::uint64_t>& census_at(Context& c, int l) {
+    auto it = cen.find(l);
+    if (it == cen.end()) it = cen.emplace(l, census(c, image(c, l))).first;
+    return it->second;
+  }
+  const DImg<std::uint16_t>& mirrored(Context& c, int l) {
+    auto it = mir.find(l);
+    if (it == mir.end()) it = mir.emplace(l, mirror_u16(c, image(c, l))).first;
+    return it->second;
+  }
+  const DImg<std::uint8_t>& mirrored_valid(Context& c, int l) {
+    auto it = mir_mask.find(l);
+    if (it == mir_mask.end()) it = mir_mask.emplace(l, mirror_u8(c, valid(c, l))).first;
+    return it->second;
+  }
+  const DImg<std::uint64_t>& mirrored_census(Context& c, int l) {
+    auto it = cen_m.find(l);
+    if (it == cen_m.end()) it = cen_m.emplace(l, census(c, mirrored(c, l))).first;
+    return it->second;
+  }
+  // Device bytes held (cache accounting).
+  [[nodiscard]] std::uint64_t bytes() const {
+    std::uint64_t b = 2 * img.pixels() + mask.pixels();
+    for (const auto& x : pyr) b += 2 * x.pixels();
+    for (const auto& x : mpyr) b += x.pixels();
+    for (const auto& [l, x] : cen) b += 8 * x.pixels();
+    for (const auto& [l, x] : cen_m) b += 8 * x.pixels();
+    for (const auto& [l, x] : mir) b += 2 * x.pixels();
+    for (const auto& [l, x] : mir_mask) b += x.pixels();
+    return b;
+  }
+};
+
 struct Job {
   const DImg<std::uint16_t>* base = nullptr;
   const DImg<std::uint16_t>* partner = nullptr;
@@ -297,24 +357,32 @@ std::size_t estimated_tile_bytes(std::uint64_t cells64, int w_, int h_, int righ
 
 int path_scanlines(int p, int w, int h) { return p < 2 ? h : (p < 4 ? w : w + h - 1); }
 
-// match_tiles (stereo_matcher.cpp:613-771) for several jobs at once: their
-// tiles share the layout pass, the 8 path launches and the selection launch.
-// Tile layout and aggregated volume S of a set of jobs (the tiles of all jobs
-// share the layout pass and the 8 path launches).
+// Tile layout of a set of jobs: the tiles of all jobs share the layout pass,
+// and chunks of consecutive tiles share the cost pass, the 8 path launches
+// and the selection launch.
 struct Aggregation {
   DBuf<JobDesc> djobs;
-  std::vector<TileDesc> td;
+  std::vector<TileDesc> td;  // cell_base counts over all tiles
   DBuf<TileDesc> dtiles;
   std::vector<Grid> grids;
   std::vector<int> tile_base;
   std::vector<std::uint64_t> tcells;
   DBuf<std::uint64_t> drow_prefix, row_start, pix;
-  DBuf<std::uint16_t> S;
-  std::uint64_t total_rows = 0, total_px = 0, total_cells = 0;
-  int max_w = 1, max_D = 16;
+  std::uint64_t total_rows = 0, total_px = 0, total_cells = 0, max_tile_cells = 0;
+  int max_w = 1, max_D = 16, max_num = 16;
 };
 
-Aggregation aggregate_jobs(Context& c, std::vector<Job>& jobs, const r3dg_options& o) {
+// Aggregated volume S of tiles [t0, t1) of a layout; the chunk's tile
+// descriptors count cells from its first tile.
+struct Chunk {
+  int t0 = 0, t1 = 0;
+  DBuf<TileDesc> dtiles;
+  DBuf<std::uint64_t> drow_prefix;
+  std::uint64_t rows = 0, cells = 0, px = 0;
+  DBuf<std::uint16_t> S;
+};
+
+Aggregation layout_jobs(Context& c, std::vector<Job>& jobs, const r3dg_options& o) {
   Aggregation A;
   const int nj = static_cast<int>(jobs.size());
   const auto lut = p2_table(o);
@@ -436,11 +504,12 @@ Aggregation aggregate_jobs(Context& c, std::vector<Job>& jobs, const r3dg_option
   }
   const int nt = static_cast<int>(td.size());
   std::uint64_t& total_cells = A.total_cells;
-  int max_num = 16;
+  int& max_num = A.max_num;
   int& max_w = A.max_w;
   int& max_D = A.max_D;
   for (int t = 0; t < nt; ++t) {
     td[std::size_t(t)].cell_base = total_cells;
+    A.max_tile_cells = std::max(A.max_tile_cells, tcells[std::size_t(t)]);
     td[std::size_t(t)].max_num = static_cast<std::int32_t>(tmax[std::size_t(t)]);
     total_cells += tcells[std::size_t(t)];
     max_num = std::max<int>(max_num, static_cast<int>(tmax[std::size_t(t)]));
@@ -450,72 +519,112 @@ Aggregation aggregate_jobs(Context& c, std::vector<Job>& jobs, const r3dg_option
   if (max_num > kMaxLabels)
     fail(R3DG_ERROR_INVALID_ARGUMENT, "stereo: more than " + std::to_string(kMaxLabels) + " labels per pixel are not supported on the GPU");
   upload(c, dtiles, td);
-  // Group prefixes of the 8 paths (4 scanlines per wave), one upload.
+  layout_timer.reset();
+  return A;
+}
+
+// Costs and the 8 path scans of tiles [t0, t1) (their rows are consecutive).
+Chunk aggregate_tiles(Context& c, Aggregation& A, int t0, int t1) {
+  Chunk ch;
+  ch.t0 = t0;
+  ch.t1 = t1;
+  const int nt = t1 - t0;
+  std::vector<TileDesc> td(A.td.begin() + t0, A.td.begin() + t1);
+  std::vector<std::uint64_t> row_prefix(static_cast<std::size_t>(nt));
+  const std::uint64_t cell0 = td.front().cell_base, row0 = td.front().row_base;
+  int max_num = 16;
+  for (int t = 0; t < nt; ++t) {
+    TileDesc& T = td[std::size_t(t)];
+    T.cell_base -= cell0;
+    row_prefix[std::size_t(t)] = T.row_base - row0;
+    ch.cells += A.tcells[std::size_t(t0 + t)];
+    ch.rows += std::uint64_t(T.h);
+    ch.px += std::uint64_t(T.w) * std::uint64_t(T.h);
+    max_num = std::max(max_num, T.max_num);
+  }
+  upload(c, ch.dtiles, td);
+  upload(c, ch.drow_prefix, row_prefix);
+  // Group prefixes of the 8 paths (kScanPerWave scanlines per wave), one upload.
   std::vector<std::uint64_t> gp(8 * (std::size_t(nt) + 1), 0);
+  std::uint64_t max_groups = 0;
   for (int p = 0; p < 8; ++p) {
     std::uint64_t* g = gp.data() + std::size_t(p) * (std::size_t(nt) + 1);
     for (int t = 0; t < nt; ++t)
       g[t + 1] = g[t] + std::uint64_t((path_scanlines(p, td[std::size_t(t)].w, td[std::size_t(t)].h) +
                                        kScanPerWave - 1) / kScanPerWave);
+    max_groups = std::max(max_groups, g[nt]);
   }
   DBuf<std::uint64_t> dgp;
   upload(c, dgp, gp);
-  A.S = DBuf<std::uint16_t>(c.arena, total_cells + kSPad);
-  DBuf<std::uint16_t>& S = A.S;
-  layout_timer.reset();
-
+  ch.S = DBuf<std::uint16_t>(c.arena, ch.cells + kSPad);
   // Scratch rows for pixels wider than the kernel's LDS rows.
-  std::uint64_t max_groups = 0;
-  for (int p = 0; p < 8; ++p) max_groups = std::max(max_groups, gp[std::size_t(p) * (std::size_t(nt) + 1) + nt]);
-  DBuf<std::uint16_t> scratch;
-  if (const std::size_t n = path_scratch_elems(max_groups, max_num); n > 0) scratch = DBuf<std::uint16_t>(c.arena, n);
+  DBuf<std::uint16_t> scratch(c.arena, std::max<std::size_t>(path_scratch_elems(max_groups, max_num), 1));
 
   // Matching costs of every cell (timed with the other cost preparation).
-  DBuf<std::uint8_t> C(c.arena, total_cells + kSPad);
+  DBuf<std::uint8_t> C(c.arena, ch.cells + kSPad);
   {
     const StageTimer timer(c, R3DG_STAGE_PREPARE);
-    launch_cost(djobs.get(), dtiles.get(), drow_prefix.get(), nt, total_rows, row_start.get(), pix.get(), C.get(),
-                c.stream);
+    launch_cost(A.djobs.get(), ch.dtiles.get(), ch.drow_prefix.get(), nt, ch.rows, A.row_start.get(), A.pix.get(),
+                C.get(), c.stream);
     check_launch();
   }
-
-  // Aggregation: 8 path launches over all tiles.
+  // Aggregation: 8 path launches over the chunk's tiles.
   {
     const StageTimer timer(c, R3DG_STAGE_AGGREGATE);
     for (int p = 0; p < 8; ++p) {
       const std::size_t o8 = std::size_t(p) * (std::size_t(nt) + 1);
-      launch_sgm_path(p, djobs.get(), dtiles.get(), dgp.get() + o8, nt, gp[o8 + std::size_t(nt)], row_start.get(),
-                      pix.get(), C.get(), S.get(), scratch.get(), max_num, p > 0, c.stream);
+      launch_sgm_path(p, A.djobs.get(), ch.dtiles.get(), dgp.get() + o8, nt, gp[o8 + std::size_t(nt)],
+                      A.row_start.get(), A.pix.get(), C.get(), ch.S.get(), scratch.get(), max_num, p > 0, c.stream);
       check_launch();
     }
   }
   c.agg_launches += 8;
   // Algorithmic bytes of the 8 launches: per cell C read 8 x 1, S written
   // 8 x 2 and read 7 x 2; per tile pixel and path its record (8) and intensity (2).
-  c.agg_bytes += 38 * total_cells + 80 * total_px;
-
-  return A;
+  c.agg_bytes += 38 * ch.cells + 80 * ch.px;
+  return ch;
 }
 
+// Tile ranges whose volumes (S, costs, scratch) fit the device memory left
+// in the context's budget, one chunk when everything fits.
+std::vector<std::array<int, 2>> chunk_tiles(const Context& c, const Aggregation& A) {
+  const std::uint64_t free_bytes = c.arena.budget() > c.arena.live() ? c.arena.budget() - c.arena.live() : 0;
+  // 3 bytes per cell (S and C) with room for scratch rows and per-tile data.
+  const std::uint64_t cap = std::max(A.max_tile_cells, free_bytes / 4);
+  std::vector<std::array<int, 2>> out;
+  const int nt = static_cast<int>(A.td.size());
+  int t0 = 0;
+  std::uint64_t cells = 0;
+  for (int t = 0; t < nt; ++t) {
+    if (t > t0 && cells + A.tcells[std::size_t(t)] > cap) {
+      out.push_back({t0, t});
+      t0 = t;
+      cells = 0;
+    }
+    cells += A.tcells[std::size_t(t)];
+  }
+  if (nt > 0) out.push_back({t0, nt});
+  return out;
+}
+
+// match_tiles (stereo_matcher.cpp:613-771) for several jobs at once.
 std::vector<TilesOut> match_tiles(Context& c, std::vector<Job>& jobs, const r3dg_options& o, SelectParams sp) {
   const int nj = static_cast<int>(jobs.size());
-  Aggregation A = aggregate_jobs(c, jobs, o);
-  const int nt = static_cast<int>(A.td.size());
+  Aggregation A = layout_jobs(c, jobs, o);
   const std::uint64_t total_px = A.total_px;
-  // Selection into per-tile maps.
   bool any_labels = false;
   for (const Job& J : jobs) any_labels = any_labels || J.labels;
   DBuf<float> tdisp(c.arena, std::max<std::uint64_t>(total_px, 1)), tconf(c.arena, std::max<std::uint64_t>(total_px, 1));
   DBuf<std::int16_t> tlab;
   if (any_labels) tlab = DBuf<std::int16_t>(c.arena, std::max<std::uint64_t>(total_px, 1));
-  {
+  for (const auto& [t0, t1] : chunk_tiles(c, A)) {
+    Chunk ch = aggregate_tiles(c, A, t0, t1);
+    // Selection into per-tile maps.
     const StageTimer timer(c, R3DG_STAGE_SELECT);
-    launch_select(A.djobs.get(), A.dtiles.get(), A.drow_prefix.get(), nt, A.total_rows, A.row_start.get(),
-                  A.pix.get(), A.S.get(), sp, A.max_w, A.max_D, tdisp.get(), tconf.get(), tlab.get(), c.stream);
+    launch_select(A.djobs.get(), ch.dtiles.get(), ch.drow_prefix.get(), t1 - t0, ch.rows, A.row_start.get(),
+                  A.pix.get(), ch.S.get(), sp, A.max_w, A.max_D, tdisp.get(), tconf.get(), tlab.get(), c.stream);
     check_launch();
   }
-  A.S.reset();
-
   std::vector<TilesOut> out(static_cast<std::size_t>(nj));
   for (int j = 0; j < nj; ++j) {
     const Job& J = jobs[std::size_t(j)];
@@ -584,109 +693,133 @@ void check_inputs(const DImg<std::uint16_t>& L, const DImg<std::uint16_t>& R, co
   if (nd <= 0) fail(R3DG_ERROR_INVALID_ARGUMENT, "stereo: no disparities to search");
 }
 
-// match_level (stereo_matcher.cpp:831-886) with the coarser level's maps.
-LevelOut match_level(Context& c, const LevelIn& in, const DImg<float>* coarse_left, const DImg<float>* coarse_right,
-                     bool need_right, int speckle, const r3dg_options& o, bool want_debug) {
-  check_inputs(*in.L, *in.R, *in.mL, *in.mR, in.nd);
-  const int wl = in.L->w, wr = in.R->w, h = in.L->h;
-  const int D = align16(in.nd);
-  const bool check_lr = o.lr_max_diff >= 0;
-  const bool full = check_lr && o.lr_full_match != 0;
-
-  std::optional<StageTimer> prep_timer(std::in_place, c, R3DG_STAGE_PREPARE);
-  DBuf<std::int32_t> lrows = intervals(c, *in.mL, wl, h);
-  DBuf<std::int32_t> rrows = intervals(c, *in.mR, wr, h);
+// Jobs of one level of one pair and the buffers they point into
+// (match_level, stereo_matcher.cpp:831-856).  Heap-allocated so that the
+// jobs' pointers stay valid while the jobs of several pairs are matched
+// together.
+struct LevelPrep {
+  LevelIn in{};
+  bool full = false, need_right = false, want_debug = false;
+  int speckle = 0;
+  DBuf<std::int32_t> lrows, rrows, mrows_base, mrows_partner;
   Ranges lranges, rranges;
-  if (coarse_left != nullptr) {
-    const StageTimer timer(c, R3DG_STAGE_RANGES);
-    lranges = ranges_from_coarse(c, *coarse_left, wl, h, *in.mL, rrows.get(), in.dmin, in.nd, o, -1, false);
-  }
+  std::vector<Job> jobs;
+};
 
-  std::vector<Job> jobs(1);
-  jobs[0].base = in.L;
-  jobs[0].partner = in.R;
-  jobs[0].cl = census(c, *in.L);
-  jobs[0].cr = census(c, *in.R);
-  jobs[0].lrows = lrows.get();
-  jobs[0].rrows = rrows.get();
-  jobs[0].rows4 = pack_rows(c, lrows.get(), rrows.get(), h);
-  jobs[0].ranges = coarse_left != nullptr ? &lranges : nullptr;
-  jobs[0].dmin = in.dmin;
-  jobs[0].D = D;
-  jobs[0].confidence = true;
-  jobs[0].labels = want_debug;
-
-  DImg<std::uint16_t> mLi, mRi;
-  DImg<std::uint8_t> mLv, mRv;
-  DBuf<std::int32_t> mrows_base, mrows_partner;
-  if (full) {
-    // Right-to-left match on the mirrored pair (stereo_matcher.cpp:842-856):
-    // base = mirror(R), partner = mirror(L), disparity offset wr - wl.
-    mLi = mirror_u16(c, *in.L);
-    mRi = mirror_u16(c, *in.R);
-    mLv = mirror_u8(c, *in.mL);
-    mRv = mirror_u8(c, *in.mR);
-    mrows_base = intervals(c, mRv, wr, h);
-    mrows_partner = intervals(c, mLv, wl, h);
-    if (coarse_right != nullptr) {
-      const StageTimer timer(c, R3DG_STAGE_RANGES);
-      rranges = ranges_from_coarse(c, *coarse_right, wr, h, *in.mR, lrows.get(), in.dmin, in.nd, o, 1, true);
-    }
-    Job b;
-    b.base = &mRi;
-    b.partner = &mLi;
-    b.cl = census(c, mRi);
-    b.cr = census(c, mLi);
-    b.lrows = mrows_base.get();
-    b.rrows = mrows_partner.get();
-    b.rows4 = pack_rows(c, mrows_base.get(), mrows_partner.get(), h);
-    b.ranges = coarse_right != nullptr ? &rranges : nullptr;
-    b.dmin = in.dmin + wr - wl;
-    b.D = D;
-    b.confidence = false;
-    jobs.push_back(std::move(b));
-  }
+SelectParams select_params(const r3dg_options& o) {
+  const bool check_lr = o.lr_max_diff >= 0;
   SelectParams sp{};
   sp.subpixel = o.subpixel;
   sp.uniqueness = o.uniqueness;
   sp.lr_max_diff = o.lr_max_diff;
-  sp.diagonal_lr = check_lr && !full ? 1 : 0;
-  prep_timer.reset();
-  std::vector<TilesOut> maps = match_tiles(c, jobs, o, sp);
-  const StageTimer filter_timer(c, R3DG_STAGE_FILTER);
+  sp.diagonal_lr = check_lr && o.lr_full_match == 0 ? 1 : 0;
+  return sp;
+}
 
+// Level l of the pair (Lv, Rv) with disparities [dmin, dmin + nd).
+std::unique_ptr<LevelPrep> prepare_level(Context& c, View& Lv, View& Rv, int l, int dmin, int nd,
+                                         const DImg<float>* coarse_left, const DImg<float>* coarse_right,
+                                         bool need_right, int speckle, const r3dg_options& o, bool want_debug) {
+  auto P = std::make_unique<LevelPrep>();
+  LevelIn& in = P->in;
+  in.L = &Lv.image(c, l);
+  in.R = &Rv.image(c, l);
+  in.mL = &Lv.valid(c, l);
+  in.mR = &Rv.valid(c, l);
+  in.dmin = dmin;
+  in.nd = nd;
+  check_inputs(*in.L, *in.R, *in.mL, *in.mR, in.nd);
+  P->need_right = need_right;
+  P->speckle = speckle;
+  P->want_debug = want_debug;
+  const int wl = in.L->w, wr = in.R->w, h = in.L->h;
+  const int D = align16(in.nd);
+  P->full = o.lr_max_diff >= 0 && o.lr_full_match != 0;
+
+  const StageTimer prep_timer(c, R3DG_STAGE_PREPARE);
+  P->lrows = intervals(c, *in.mL, wl, h);
+  P->rrows = intervals(c, *in.mR, wr, h);
+  if (coarse_left != nullptr) {
+    const StageTimer timer(c, R3DG_STAGE_RANGES);
+    P->lranges = ranges_from_coarse(c, *coarse_left, wl, h, *in.mL, P->rrows.get(), in.dmin, in.nd, o, -1, false);
+  }
+  P->jobs.resize(1);
+  Job& a = P->jobs[0];
+  a.base = in.L;
+  a.partner = in.R;
+  a.cl = view_of(Lv.census_at(c, l));
+  a.cr = view_of(Rv.census_at(c, l));
+  a.lrows = P->lrows.get();
+  a.rrows = P->rrows.get();
+  a.rows4 = pack_rows(c, P->lrows.get(), P->rrows.get(), h);
+  a.ranges = coarse_left != nullptr ? &P->lranges : nullptr;
+  a.dmin = in.dmin;
+  a.D = D;
+  a.confidence = true;
+  a.labels = want_debug;
+  if (P->full) {
+    // Right-to-left match on the mirrored pair (stereo_matcher.cpp:842-856):
+    // base = mirror(R), partner = mirror(L), disparity offset wr - wl.
+    P->mrows_base = intervals(c, Rv.mirrored_valid(c, l), wr, h);
+    P->mrows_partner = intervals(c, Lv.mirrored_valid(c, l), wl, h);
+    if (coarse_right != nullptr) {
+      const StageTimer timer(c, R3DG_STAGE_RANGES);
+      P->rranges = ranges_from_coarse(c, *coarse_right, wr, h, *in.mR, P->lrows.get(), in.dmin, in.nd, o, 1, true);
+    }
+    Job b;
+    b.base = &Rv.mirrored(c, l);
+    b.partner = &Lv.mirrored(c, l);
+    b.cl = view_of(Rv.mirrored_census(c, l));
+    b.cr = view_of(Lv.mirrored_census(c, l));
+    b.lrows = P->mrows_base.get();
+    b.rrows = P->mrows_partner.get();
+    b.rows4 = pack_rows(c, P->mrows_base.get(), P->mrows_partner.get(), h);
+    b.ranges = coarse_right != nullptr ? &P->rranges : nullptr;
+    b.dmin = in.dmin + wr - wl;
+    b.D = D;
+    b.confidence = false;
+    P->jobs.push_back(std::move(b));
+  }
+  return P;
+}
+
+// LR checks and post filters of a matched level (stereo_matcher.cpp:857-886);
+// maps holds the tile maps of P's jobs.
+LevelOut finish_level(Context& c, LevelPrep& P, std::vector<TilesOut>& maps, const r3dg_options& o) {
+  const StageTimer filter_timer(c, R3DG_STAGE_FILTER);
+  const int wl = P.in.L->w, wr = P.in.R->w, h = P.in.L->h;
   LevelOut out;
   out.left = std::move(maps[0]);
-  if (want_debug) {
+  if (P.want_debug) {
     out.lr_mask = alloc_img<std::uint8_t>(c, wl, h);
     launch_fill_u8(out.lr_mask.p, out.lr_mask.pixels(), 0, c.stream);
     check_launch();
   }
-  if (full) {
+  if (P.full) {
     out.left.cells += maps[1].cells;
     out.left.tile_pixels += maps[1].tile_pixels;
     out.right = alloc_img<float>(c, wr, h);
     launch_unmirror_right(maps[1].disp.p, wr, h, wl, out.right.p, c.stream);
     check_launch();
     DBuf<std::uint8_t> rbad;
-    if (need_right) {
+    if (P.need_right) {
       // Both checks compare the unchecked maps (stereo_matcher.cpp:864-876).
       rbad = DBuf<std::uint8_t>(c.arena, out.right.pixels());
       launch_lr_check(out.right.p, wr, h, out.left.disp.p, wl, 1, o.lr_max_diff, rbad.get(), false, c.stream);
       check_launch();
     }
     launch_lr_check(out.left.disp.p, wl, h, out.right.p, wr, -1, o.lr_max_diff,
-                    want_debug ? out.lr_mask.p : nullptr, true, c.stream);
+                    P.want_debug ? out.lr_mask.p : nullptr, true, c.stream);
     check_launch();
-    if (need_right) {
+    if (P.need_right) {
       launch_apply_mask(out.right.p, out.right.pixels(), rbad.get(), c.stream);
       check_launch();
-      post_filter(c, out.right, o, speckle);
+      post_filter(c, out.right, o, P.speckle);
     } else {
       out.right = DImg<float>{};
     }
   }
-  post_filter(c, out.left.disp, o, speckle);
+  post_filter(c, out.left.disp, o, P.speckle);
   return out;
 }
 
@@ -698,53 +831,110 @@ struct MatchResult {
   int tiles0 = 0;
 };
 
-// match_stereo (stereo_matcher.cpp:1023-1095).
-MatchResult match_stereo(Context& c, const DImg<std::uint16_t>& left, const DImg<std::uint16_t>& right,
-                         const DImg<std::uint8_t>& lvalid, const DImg<std::uint8_t>& rvalid, int dmin, int nd,
-                         const r3dg_options& o, bool want_debug) {
-  check_options(o);
-  if (nd <= 0) fail(R3DG_ERROR_INVALID_ARGUMENT, "stereo: no disparities to search");
-  check_inputs(left, right, lvalid, rvalid, nd);
-  const int levels = o.levels >= 0 ? o.levels : automatic_levels(left.w, left.h, nd);
-  const bool lr_full = o.lr_max_diff >= 0 && o.lr_full_match != 0;
-  const int dmax = dmin + nd - 1;
-  std::vector<DImg<std::uint16_t>> lp, rp;
-  std::vector<DImg<std::uint8_t>> lvp, rvp;
-  std::optional<StageTimer> pyr_timer(std::in_place, c, R3DG_STAGE_PYRAMID);
-  for (int l = 1; l <= levels; ++l) {
-    lp.push_back(pyramid_u16(c, l == 1 ? left : lp.back()));
-    rp.push_back(pyramid_u16(c, l == 1 ? right : rp.back()));
-    if (!lvalid.empty()) lvp.push_back(pyramid_u8(c, l == 1 ? lvalid : lvp.back(), true));
-    if (!rvalid.empty()) rvp.push_back(pyramid_u8(c, l == 1 ? rvalid : rvp.back(), true));
-  }
-  pyr_timer.reset();
-  MatchResult res;
-  res.levels = levels;
+// One pair through the levels of match_stereo (stereo_matcher.cpp:1023-1095).
+struct PairRun {
+  View* lv = nullptr;
+  View* rv = nullptr;
+  int dmin = 0, nd = 0, levels = 0;
+  bool lr_full = false, want_debug = false;
+  int next = 0;  // next level to match (levels down to 0)
   LevelOut coarse;
-  for (int l = levels; l >= 0; --l) {
-    if (c.cancel.load() != 0) fail(R3DG_ERROR_CANCELLED, "cancelled");
-    const std::size_t i = std::size_t(l - 1);
-    LevelIn in{};
-    in.L = l == 0 ? &left : &lp[i];
-    in.R = l == 0 ? &right : &rp[i];
-    in.mL = l == 0 || lvalid.empty() ? &lvalid : &lvp[i];
-    in.mR = l == 0 || rvalid.empty() ? &rvalid : &rvp[i];
-    in.dmin = floor_shift(dmin, l);
-    in.nd = align16(ceil_shift(dmax, l) - in.dmin + 1);
-    const int speckle = o.speckle_size >> (2 * l);
-    if (l == levels) {
-      coarse = match_level(c, in, nullptr, nullptr, lr_full && l > 0, speckle, o, l == 0 && want_debug);
-    } else {
-      LevelOut prev = std::move(coarse);
-      coarse = match_level(c, in, &prev.left.disp, lr_full ? &prev.right : nullptr, lr_full && l > 0, speckle, o,
-                           l == 0 && want_debug);
-    }
-    res.cells += coarse.left.cells;
-    res.tile_pixels += coarse.left.tile_pixels;
-    if (l == 0) res.tiles0 = coarse.left.tiles;
+  MatchResult res;
+};
+
+// Checks and the image pyramids of a pair.
+void start_pair(Context& c, PairRun& P, const r3dg_options& o) {
+  if (P.nd <= 0) fail(R3DG_ERROR_INVALID_ARGUMENT, "stereo: no disparities to search");
+  check_inputs(P.lv->img, P.rv->img, P.lv->mask, P.rv->mask, P.nd);
+  P.levels = o.levels >= 0 ? o.levels : automatic_levels(P.lv->img.w, P.lv->img.h, P.nd);
+  P.lr_full = o.lr_max_diff >= 0 && o.lr_full_match != 0;
+  P.res.levels = P.levels;
+  P.next = P.levels;
+  const StageTimer pyr_timer(c, R3DG_STAGE_PYRAMID);
+  P.lv->image(c, P.levels);
+  P.rv->image(c, P.levels);
+  P.lv->valid(c, P.levels);
+  P.rv->valid(c, P.levels);
+}
+
+std::unique_ptr<LevelPrep> prepare_pair_level(Context& c, PairRun& P, int l, const r3dg_options& o) {
+  const int dmin = floor_shift(P.dmin, l);
+  const int nd = align16(ceil_shift(P.dmin + P.nd - 1, l) - dmin + 1);
+  const int speckle = o.speckle_size >> (2 * l);
+  const bool top = l == P.levels;
+  return prepare_level(c, *P.lv, *P.rv, l, dmin, nd, top ? nullptr : &P.coarse.left.disp,
+                       top || !P.lr_full ? nullptr : &P.coarse.right, P.lr_full && l > 0, speckle, o,
+                       l == 0 && P.want_debug);
+}
+
+void finish_pair_level(Context& c, PairRun& P, int l, LevelPrep& prep, std::vector<TilesOut>& maps,
+                       const r3dg_options& o) {
+  P.coarse = finish_level(c, prep, maps, o);
+  P.res.cells += P.coarse.left.cells;
+  P.res.tile_pixels += P.coarse.left.tile_pixels;
+  P.next = l - 1;
+  if (l == 0) {
+    P.res.tiles0 = P.coarse.left.tiles;
+    P.res.final = std::move(P.coarse);
   }
-  res.final = std::move(coarse);
-  return res;
+}
+
+// One step of several pairs: the next level of each, their jobs matched
+// together (one layout, shared cost, path and selection launches).
+void match_step(Context& c, const std::vector<PairRun*>& part, const r3dg_options& o) {
+  std::vector<std::unique_ptr<LevelPrep>> preps;
+  std::vector<Job> jobs;
+  for (PairRun* P : part) {
+    preps.push_back(prepare_pair_level(c, *P, P->next, o));
+    for (Job& J : preps.back()->jobs) jobs.push_back(std::move(J));
+  }
+  std::vector<TilesOut> maps = match_tiles(c, jobs, o, select_params(o));
+  std::size_t k = 0;
+  for (std::size_t q = 0; q < part.size(); ++q) {
+    const std::size_t n = preps[q]->full ? 2 : 1;
+    std::vector<TilesOut> mine;
+    for (std::size_t j = 0; j < n; ++j) mine.push_back(std::move(maps[k + j]));
+    k += n;
+    finish_pair_level(c, *part[q], part[q]->next, *preps[q], mine, o);
+  }
+}
+
+// match_stereo (stereo_matcher.cpp:1023-1095).
+MatchResult match_stereo(Context& c, View& Lv, View& Rv, int dmin, int nd, const r3dg_options& o,
+                         bool want_debug) {
+  check_options(o);
+  PairRun P;
+  P.lv = &Lv;
+  P.rv = &Rv;
+  P.dmin = dmin;
+  P.nd = nd;
+  P.want_debug = want_debug;
+  start_pair(c, P, o);
+  while (P.next >= 0) {
+    if (c.cancel.load() != 0) fail(R3DG_ERROR_CANCELLED, "cancelled");
+    match_step(c, {&P}, o);
+  }
+  return std::move(P.res);
+}
+
+// match_stereo of several pairs with their levels in lockstep, finest levels
+// together: at each step the jobs of every pair that has the level share one
+// layout, the cost passes, the path launches and the selection launches.
+// Each pair's maps are the same as when it is matched alone (tiles, ranges and
+// every per-pixel step depend only on the pair).
+void match_batch(Context& c, const std::vector<PairRun*>& pairs, const r3dg_options& o) {
+  int top = 0;
+  for (PairRun* P : pairs) {
+    start_pair(c, *P, o);
+    top = std::max(top, P->levels);
+  }
+  for (int l = top; l >= 0; --l) {
+    if (c.cancel.load() != 0) fail(R3DG_ERROR_CANCELLED, "cancelled");
+    std::vector<PairRun*> part;
+    for (PairRun* P : pairs)
+      if (P->next == l) part.push_back(P);
+    if (!part.empty()) match_step(c, part, o);
+  }
 }
 
 // ------------------------------------------------------------------ C ABI glue
@@ -813,6 +1003,211 @@ const r3dg_options& opts_or_default(const r3dg_options* o, r3dg_options& tmp) {
   return tmp;
 }
 
+// ------------------------------------------------------------------ serving
+using ViewCache = std::unordered_map<std::uint64_t, std::shared_ptr<View>>;
+
+ViewCache& view_cache(Context& c) {
+  if (!c.views) c.views = std::make_shared<ViewCache>();
+  return *static_cast<ViewCache*>(c.views.get());
+}
+
+// Device bytes a pair is expected to need: per-pixel buffers (pyramids,
+// Census, mirrored copies, maps) and its largest level's volumes (S and
+// costs, 3 bytes per cell) with about 48 labels per pixel when ranges come
+// from a coarser level.  Volumes beyond the memory left are chunked anyway.
+std::uint64_t pair_footprint(const r3dg_pair& q, const r3dg_options& o) {
+  const std::uint64_t wl = std::uint64_t(std::max(q.left.width, 0)), wr = std::uint64_t(std::max(q.right.width, 0));
+  const std::uint64_t h = std::uint64_t(std::max(q.left.height, 0));
+  const int levels = o.levels >= 0 ? o.levels : automatic_levels(q.left.width, q.left.height, q.num_disparities);
+  const std::uint64_t labels = levels > 0 ? 48 : std::uint64_t(align16(std::max(q.num_disparities, 1)));
+  return 96 * (wl + wr) * h + 3 * 12 * wl * h * labels / 5;
+}
+
+struct Request {
+  r3dg_pair* pair = nullptr;
+  std::uint64_t left_view = 0, right_view = 0;
+  std::uint64_t footprint = 0;
+  std::chrono::steady_clock::time_point submitted;
+  int32_t status = R3DG_PENDING;
+  // Worker-side state while admitted.
+  std::shared_ptr<View> lv, rv;
+  PairRun run;
+};
+
+// Request loop of a context (r3dg_server_start).
+class Server {
+ public:
+  Server(Context& c, const r3dg_options& o, std::uint64_t budget) : c_(c), o_(o), budget_(budget) {
+    worker_ = std::thread([this] { loop(); });
+  }
+  ~Server() { stop(); }
+
+  void stop() {
+    {
+      const std::lock_guard<std::mutex> lock(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    if (worker_.joinable()) worker_.join();
+  }
+
+  std::uint64_t submit(r3dg_pair* pair, std::uint64_t lv, std::uint64_t rv) {
+    auto r = std::make_unique<Request>();
+    r->pair = pair;
+    r->left_view = lv;
+    r->right_view = rv;
+    r->footprint = pair_footprint(*pair, o_);
+    r->submitted = std::chrono::steady_clock::now();
+    pair->status = R3DG_PENDING;
+    std::uint64_t t;
+    {
+      const std::lock_guard<std::mutex> lock(m_);
+      if (stop_) fail(R3DG_ERROR_INVALID_ARGUMENT, "serve: the request loop is stopping");
+      t = next_++;
+      pending_.push_back(t);
+      reqs_.emplace(t, std::move(r));
+    }
+    cv_.notify_all();
+    return t;
+  }
+
+  // R3DG_PENDING, or the finished pair's status (the ticket is released).
+  int32_t wait(std::uint64_t t, double timeout) {
+    std::unique_lock<std::mutex> lock(m_);
+    auto it = reqs_.find(t);
+    if (it == reqs_.end()) fail(R3DG_ERROR_NOT_FOUND, "serve: unknown ticket " + std::to_string(t));
+    auto ready = [&] { return it->second->status != R3DG_PENDING; };
+    if (timeout < 0) {
+      done_cv_.wait(lock, ready);
+    } else if (timeout > 0) {
+      done_cv_.wait_for(lock, std::chrono::duration<double>(timeout), ready);
+    }
+    if (!ready()) return R3DG_PENDING;
+    const int32_t st = it->second->status;
+    reqs_.erase(it);
+    return st;
+  }
+
+ private:
+  void complete(Request* r, int32_t status) {
+    r->run = PairRun{};
+    r->lv.reset();
+    r->rv.reset();
+    {
+      const std::lock_guard<std::mutex> lock(m_);
+      r->pair->info.seconds =
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - r->submitted).count();
+      r->pair->status = status;
+      r->status = status;
+      in_use_ -= r->footprint;
+    }
+    done_cv_.notify_all();
+  }
+
+  std::shared_ptr<View> view_for(std::uint64_t id, const r3dg_image& img, const r3dg_image& mask, const char* name) {
+    if (id != 0) {
+      ViewCache& vc = view_cache(c_);
+      auto it = vc.find(id);
+      if (it == vc.end()) fail(R3DG_ERROR_NOT_FOUND, std::string("serve: unknown ") + name + " view " + std::to_string(id));
+      return it->second;
+    }
+    auto v = std::make_shared<View>();
+    v->img = input_img<std::uint16_t>(c_, img, name);
+    v->mask = input_img<std::uint8_t>(c_, mask, "valid");
+    return v;
+  }
+
+  void loop() {
+    std::vector<Request*> active;
+    for (;;) {
+      std::vector<Request*> admit;
+      {
+        std::unique_lock<std::mutex> lock(m_);
+        cv_.wait(lock, [&] { return stop_ || !pending_.empty() || !active.empty(); });
+        if (stop_ && pending_.empty() && active.empty()) return;
+        // Admission by estimated footprint; a pair that fits nowhere runs alone.
+        while (!pending_.empty()) {
+          Request* r = reqs_.at(pending_.front()).get();
+          if ((!active.empty() || !admit.empty()) && in_use_ + r->footprint > budget_) break;
+          in_use_ += r->footprint;
+          admit.push_back(r);
+          pending_.pop_front();
+        }
+      }
+      const std::lock_guard<std::mutex> ctx_lock(c_.mutex);
+      (void)hipSetDevice(c_.device);
+      for (Request* r : admit) {
+        const int32_t st = guarded(&c_, [&] {
+          r3dg_pair& q = *r->pair;
+          std::memset(&q.info, 0, sizeof(q.info));
+          if (view_empty(q.disparity)) fail(R3DG_ERROR_INVALID_ARGUMENT, "match: empty disparity output");
+          r->lv = view_for(r->left_view, q.left, q.left_valid, "left");
+          r->rv = view_for(r->right_view, q.right, q.right_valid, "right");
+          r->run = PairRun{};
+          r->run.lv = r->lv.get();
+          r->run.rv = r->rv.get();
+          r->run.dmin = q.disparity_min;
+          r->run.nd = q.num_disparities;
+          start_pair(c_, r->run, o_);
+        });
+        if (st == R3DG_OK) {
+          active.push_back(r);
+        } else {
+          complete(r, st);
+        }
+      }
+      if (active.empty()) continue;
+      std::vector<PairRun*> runs;
+      for (Request* r : active) runs.push_back(&r->run);
+      const int32_t st = guarded(&c_, [&] {
+        match_step(c_, runs, o_);
+        for (Request* r : active) {
+          if (r->run.next >= 0) continue;
+          r3dg_pair& q = *r->pair;
+          const TilesOut& m = r->run.res.final.left;
+          output_img(c_, q.disparity, m.disp.p, m.disp.w, m.disp.h, m.disp.s, "disparity");
+          if (!view_empty(q.confidence))
+            output_img(c_, q.confidence, m.conf.p, m.conf.w, m.conf.h, m.conf.s, "confidence");
+          q.info.disparity_min = q.disparity_min;
+          q.info.num_disparities = align16(q.num_disparities);
+          q.info.levels = r->run.res.levels;
+          q.info.tiles_level0 = r->run.res.tiles0;
+          q.info.cells = r->run.res.cells;
+          q.info.tile_pixels = r->run.res.tile_pixels;
+        }
+        sync(c_);
+      });
+      std::vector<Request*> still;
+      for (Request* r : active) {
+        if (st != R3DG_OK) {
+          complete(r, st);
+        } else if (r->run.next < 0) {
+          complete(r, R3DG_OK);
+        } else {
+          still.push_back(r);
+        }
+      }
+      active.swap(still);
+    }
+  }
+
+  Context& c_;
+  const r3dg_options o_;
+  const std::uint64_t budget_;
+  std::mutex m_;
+  std::condition_variable cv_, done_cv_;
+  std::deque<std::uint64_t> pending_;
+  std::unordered_map<std::uint64_t, std::unique_ptr<Request>> reqs_;
+  std::uint64_t next_ = 1, in_use_ = 0;
+  bool stop_ = false;
+  std::thread worker_;
+};
+
+Server* server_of(Context& c) {
+  if (!c.server) fail(R3DG_ERROR_INVALID_ARGUMENT, "serve: no request loop (r3dg_server_start)");
+  return static_cast<Server*>(c.server.get());
+}
+
 }  // namespace
 }  // namespace r3dg
 
@@ -867,9 +1262,11 @@ int32_t r3dg_ctx_create(int32_t device, uint64_t hbm_budget, r3dg_ctx** out) {
 int32_t r3dg_ctx_destroy(r3dg_ctx* h) {
   if (h == nullptr) return R3DG_OK;
   auto* c = reinterpret_cast<Context*>(h);
+  (void)r3dg_server_stop(h);
   return guarded(nullptr, [&] {
     hip_check(hipSetDevice(c->device), "hipSetDevice");
     (void)hipStreamSynchronize(c->stream);
+    c->views.reset();
     c->arena.trim();
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1002,10 +1399,11 @@ int32_t r3dg_cost_volume(r3dg_ctx* h, r3dg_image left, r3dg_image right, int32_t
     jobs[0].ranges = has ? &rg : nullptr;
     jobs[0].dmin = disparity_min;
     jobs[0].D = D;
-    Aggregation A = aggregate_jobs(c, jobs, one);
+    Aggregation A = layout_jobs(c, jobs, one);
     if (A.total_cells != out_cells) fail(R3DG_ERROR_INTERNAL, "cost_volume: cell count mismatch");
+    const Chunk ch = aggregate_tiles(c, A, 0, static_cast<int>(A.td.size()));
     const hipMemcpyKind kind = out_memory == R3DG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-    if (out_cells > 0) hip_check(hipMemcpyAsync(out, A.S.get(), out_cells * 2, kind, c.stream), "download");
+    if (out_cells > 0) hip_check(hipMemcpyAsync(out, ch.S.get(), out_cells * 2, kind, c.stream), "download");
     sync(c);
   });
 }
@@ -1018,10 +1416,11 @@ int32_t r3dg_match(r3dg_ctx* h, r3dg_image left, r3dg_image right, r3dg_image le
     r3dg_options tmp;
     const r3dg_options& o = opts_or_default(options, tmp);
     check_options(o);
-    DImg<std::uint16_t> L = input_img<std::uint16_t>(c, left, "left");
-    DImg<std::uint16_t> R = input_img<std::uint16_t>(c, right, "right");
-    DImg<std::uint8_t> mL = input_img<std::uint8_t>(c, left_valid, "left_valid");
-    DImg<std::uint8_t> mR = input_img<std::uint8_t>(c, right_valid, "right_valid");
+    View Lv, Rv;
+    Lv.img = input_img<std::uint16_t>(c, left, "left");
+    Rv.img = input_img<std::uint16_t>(c, right, "right");
+    Lv.mask = input_img<std::uint8_t>(c, left_valid, "left_valid");
+    Rv.mask = input_img<std::uint8_t>(c, right_valid, "right_valid");
     if (view_empty(disparity)) fail(R3DG_ERROR_INVALID_ARGUMENT, "match: empty disparity output");
     const bool want_debug = debug != nullptr && (!view_empty(debug->labels) || !view_empty(debug->lr_mask));
     c.agg_launches = 0;
@@ -1029,7 +1428,7 @@ int32_t r3dg_match(r3dg_ctx* h, r3dg_image left, r3dg_image right, r3dg_image le
     double stage_seconds[R3DG_STAGES] = {};
     c.collect(stage_seconds, R3DG_STAGES);  // drop marks of earlier calls
     std::fill(stage_seconds, stage_seconds + R3DG_STAGES, 0.0);
-    MatchResult r = match_stereo(c, L, R, mL, mR, disparity_min, num_disparities, o, want_debug);
+    MatchResult r = match_stereo(c, Lv, Rv, disparity_min, num_disparities, o, want_debug);
     const TilesOut& m = r.final.left;
     output_img(c, disparity, m.disp.p, m.disp.w, m.disp.h, m.disp.s, "disparity");
     if (!view_empty(confidence)) output_img(c, confidence, m.conf.p, m.conf.w, m.conf.h, m.conf.s, "confidence");
@@ -1055,6 +1454,205 @@ int32_t r3dg_match(r3dg_ctx* h, r3dg_image left, r3dg_image right, r3dg_image le
     } else {
       c.collect(stage_seconds, R3DG_STAGES);
     }
+  });
+}
+
+int32_t r3dg_match_batch(r3dg_ctx* h, r3dg_pair* pairs, int32_t count, const r3dg_options* options) {
+  int32_t first_error = R3DG_OK;
+  std::string first_message;
+  const int32_t rc = with_ctx(h, [&](Context& c) {
+    const auto t0 = std::chrono::steady_clock::now();
+    r3dg_options tmp;
+    const r3dg_options& o = opts_or_default(options, tmp);
+    check_options(o);
+    if (count < 0 || (count > 0 && pairs == nullptr)) fail(R3DG_ERROR_INVALID_ARGUMENT, "match_batch: no pairs");
+    struct Inputs {
+      View L, R;
+      PairRun run;
+    };
+    std::vector<std::unique_ptr<Inputs>> in(static_cast<std::size_t>(count));
+    std::vector<int> ok;
+    for (int i = 0; i < count; ++i) {
+      r3dg_pair& q = pairs[i];
+      q.status = R3DG_OK;
+      std::memset(&q.info, 0, sizeof(q.info));
+      try {
+        auto x = std::make_unique<Inputs>();
+        x->L.img = input_img<std::uint16_t>(c, q.left, "left");
+        x->R.img = input_img<std::uint16_t>(c, q.right, "right");
+        x->L.mask = input_img<std::uint8_t>(c, q.left_valid, "left_valid");
+        x->R.mask = input_img<std::uint8_t>(c, q.right_valid, "right_valid");
+        if (view_empty(q.disparity)) fail(R3DG_ERROR_INVALID_ARGUMENT, "match: empty disparity output");
+        if (q.num_disparities <= 0) fail(R3DG_ERROR_INVALID_ARGUMENT, "stereo: no disparities to search");
+        check_inputs(x->L.img, x->R.img, x->L.mask, x->R.mask, q.num_disparities);
+        PairRun& P = x->run;
+        P.lv = &x->L;
+        P.rv = &x->R;
+        P.dmin = q.disparity_min;
+        P.nd = q.num_disparities;
+        in[std::size_t(i)] = std::move(x);
+        ok.push_back(i);
+      } catch (const Failure& e) {
+        q.status = e.status;
+        if (first_error == R3DG_OK) {
+          first_error = e.status;
+          first_message = e.what();
+        }
+      }
+    }
+    c.agg_launches = 0;
+    c.agg_bytes = 0;
+    double stage_seconds[R3DG_STAGES] = {};
+    c.collect(stage_seconds, R3DG_STAGES);  // drop marks of earlier calls
+    std::fill(stage_seconds, stage_seconds + R3DG_STAGES, 0.0);
+    // Groups whose per-pixel buffers (pyramids, Census, maps: about 96 bytes
+    // per input pixel) take at most half of the memory left; the volumes are
+    // chunked to the rest by match_tiles.
+    const std::uint64_t room = c.arena.budget() > c.arena.live() ? (c.arena.budget() - c.arena.live()) / 2 : 0;
+    std::size_t g0 = 0;
+    while (g0 < ok.size()) {
+      std::size_t g1 = g0;
+      std::uint64_t need = 0;
+      while (g1 < ok.size()) {
+        const r3dg_pair& q = pairs[ok[g1]];
+        const std::uint64_t est = 96 * (std::uint64_t(q.left.width) + std::uint64_t(q.right.width)) *
+                                  std::uint64_t(q.left.height);
+        if (g1 > g0 && need + est > room) break;
+        need += est;
+        ++g1;
+      }
+      std::vector<PairRun*> group;
+      for (std::size_t k = g0; k < g1; ++k) group.push_back(&in[std::size_t(ok[k])]->run);
+      match_batch(c, group, o);
+      for (std::size_t k = g0; k < g1; ++k) {
+        const int i = ok[k];
+        r3dg_pair& q = pairs[i];
+        Inputs& x = *in[std::size_t(i)];
+        const TilesOut& m = x.run.res.final.left;
+        output_img(c, q.disparity, m.disp.p, m.disp.w, m.disp.h, m.disp.s, "disparity");
+        if (!view_empty(q.confidence)) output_img(c, q.confidence, m.conf.p, m.conf.w, m.conf.h, m.conf.s, "confidence");
+        q.info.disparity_min = q.disparity_min;
+        q.info.num_disparities = align16(q.num_disparities);
+        q.info.levels = x.run.res.levels;
+        q.info.tiles_level0 = x.run.res.tiles0;
+        q.info.cells = x.run.res.cells;
+        q.info.tile_pixels = x.run.res.tile_pixels;
+        in[std::size_t(i)].reset();
+      }
+      sync(c);
+      g0 = g1;
+    }
+    c.collect(stage_seconds, R3DG_STAGES);
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (const int i : ok) {
+      pairs[i].info.seconds = secs;
+      pairs[i].info.aggregate_launches = c.agg_launches;
+      pairs[i].info.aggregate_bytes = c.agg_bytes;
+      for (int k = 0; k < R3DG_STAGES; ++k) pairs[i].info.stage_seconds[k] = stage_seconds[k];
+    }
+    if (first_error != R3DG_OK) fail(first_error, first_message);
+  });
+  if (rc != R3DG_OK && first_error == R3DG_OK && pairs != nullptr) {
+    for (int i = 0; i < count; ++i)
+      if (pairs[i].status == R3DG_OK) pairs[i].status = rc;
+  }
+  return rc;
+}
+
+int32_t r3dg_server_start(r3dg_ctx* h, const r3dg_options* options, uint64_t hbm_budget) {
+  return with_ctx(h, [&](Context& c) {
+    r3dg_options tmp;
+    const r3dg_options& o = opts_or_default(options, tmp);
+    check_options(o);
+    if (c.server) fail(R3DG_ERROR_INVALID_ARGUMENT, "serve: the request loop is already running");
+    const std::uint64_t budget = hbm_budget != 0 ? hbm_budget : c.arena.budget() / 10 * 8;
+    c.server = std::make_shared<Server>(c, o, budget);
+  });
+}
+
+int32_t r3dg_server_stop(r3dg_ctx* h) {
+  if (h == nullptr) return with_ctx(h, [](Context&) {});
+  auto* c = reinterpret_cast<Context*>(h);
+  std::shared_ptr<void> srv;
+  {
+    const std::lock_guard<std::mutex> lock(c->mutex);
+    srv = std::move(c->server);
+  }
+  // The loop takes the context lock for its steps: stop it without holding it.
+  if (srv) static_cast<Server*>(srv.get())->stop();
+  return R3DG_OK;
+}
+
+int32_t r3dg_submit(r3dg_ctx* h, r3dg_pair* pair, uint64_t left_view, uint64_t right_view, uint64_t* ticket) {
+  if (h == nullptr || pair == nullptr || ticket == nullptr) return with_ctx(nullptr, [](Context&) {});
+  auto* c = reinterpret_cast<Context*>(h);
+  return guarded(nullptr, [&] {
+    std::shared_ptr<void> srv;
+    {
+      const std::lock_guard<std::mutex> lock(c->mutex);
+      srv = c->server;
+    }
+    if (!srv) fail(R3DG_ERROR_INVALID_ARGUMENT, "serve: no request loop (r3dg_server_start)");
+    *ticket = static_cast<Server*>(srv.get())->submit(pair, left_view, right_view);
+  });
+}
+
+namespace {
+int32_t wait_ticket(r3dg_ctx* h, uint64_t ticket, double timeout) {
+  if (h == nullptr) return with_ctx(nullptr, [](Context&) {});
+  auto* c = reinterpret_cast<Context*>(h);
+  int32_t st = R3DG_OK;
+  const int32_t rc = guarded(nullptr, [&] {
+    std::shared_ptr<void> srv;
+    {
+      const std::lock_guard<std::mutex> lock(c->mutex);
+      srv = c->server;
+    }
+    if (!srv) fail(R3DG_ERROR_INVALID_ARGUMENT, "serve: no request loop (r3dg_server_start)");
+    st = static_cast<Server*>(srv.get())->wait(ticket, timeout);
+  });
+  return rc != R3DG_OK ? rc : st;
+}
+}  // namespace
+
+int32_t r3dg_poll(r3dg_ctx* h, uint64_t ticket) { return wait_ticket(h, ticket, 0.0); }
+
+int32_t r3dg_wait(r3dg_ctx* h, uint64_t ticket, double timeout_seconds) {
+  return wait_ticket(h, ticket, timeout_seconds);
+}
+
+int32_t r3dg_view_put(r3dg_ctx* h, uint64_t view, r3dg_image image, r3dg_image valid) {
+  return with_ctx(h, [&](Context& c) {
+    if (view == 0) fail(R3DG_ERROR_INVALID_ARGUMENT, "view: id 0 is reserved");
+    if (view_empty(image)) fail(R3DG_ERROR_INVALID_ARGUMENT, "view: empty image");
+    auto v = std::make_shared<View>();
+    v->img = input_img<std::uint16_t>(c, image, "view");
+    v->mask = input_img<std::uint8_t>(c, valid, "view valid");
+    if (!v->mask.empty() && (v->mask.w != v->img.w || v->mask.h != v->img.h))
+      fail(R3DG_ERROR_INVALID_ARGUMENT, "view: mask size differs from the image");
+    if (image.memory == R3DG_DEVICE) {
+      // Own a copy: the caller's device memory may change after the call.
+      DImg<std::uint16_t> own = alloc_img<std::uint16_t>(c, v->img.w, v->img.h);
+      hip_check(hipMemcpy2DAsync(own.p, std::size_t(own.s) * 2, v->img.p, std::size_t(v->img.s) * 2,
+                                 std::size_t(v->img.w) * 2, std::size_t(v->img.h), hipMemcpyDeviceToDevice, c.stream),
+                "copy");
+      v->img = std::move(own);
+    }
+    if (valid.memory == R3DG_DEVICE && !v->mask.empty()) {
+      DImg<std::uint8_t> own = alloc_img<std::uint8_t>(c, v->mask.w, v->mask.h);
+      hip_check(hipMemcpy2DAsync(own.p, std::size_t(own.s), v->mask.p, std::size_t(v->mask.s), std::size_t(v->mask.w),
+                                 std::size_t(v->mask.h), hipMemcpyDeviceToDevice, c.stream),
+                "copy");
+      v->mask = std::move(own);
+    }
+    sync(c);
+    view_cache(c)[view] = std::move(v);
+  });
+}
+
+int32_t r3dg_view_evict(r3dg_ctx* h, uint64_t view) {
+  return with_ctx(h, [&](Context& c) {
+    if (view_cache(c).erase(view) == 0) fail(R3DG_ERROR_NOT_FOUND, "view: unknown id " + std::to_string(view));
   });
 }
 

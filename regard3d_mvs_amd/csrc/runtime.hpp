@@ -7,6 +7,7 @@
 #include <atomic>
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -52,6 +53,7 @@ class Arena {
     if (it != free_.end() && it->first <= 2 * size) {
       void* p = it->second;
       live_[p] = it->first;
+      live_bytes_ += it->first;
       free_.erase(it);
       return p;
     }
@@ -71,6 +73,7 @@ class Arena {
     hip_check(e, "hipMalloc");
     held_ += size;
     live_[p] = size;
+    live_bytes_ += size;
     return p;
   }
 
@@ -79,6 +82,7 @@ class Arena {
     auto it = live_.find(p);
     if (it == live_.end()) return;
     free_.emplace(it->second, p);
+    live_bytes_ -= it->second;
     live_.erase(it);
   }
 
@@ -91,6 +95,8 @@ class Arena {
   }
 
   [[nodiscard]] std::uint64_t held() const { return held_; }
+  // Bytes of blocks in use (held minus the cached free blocks).
+  [[nodiscard]] std::uint64_t live() const { return live_bytes_; }
   [[nodiscard]] std::uint64_t budget() const { return budget_; }
 
  private:
@@ -101,6 +107,7 @@ class Arena {
   std::multimap<std::size_t, void*> free_;
   std::unordered_map<void*, std::size_t> live_;
   std::uint64_t held_ = 0;
+  std::uint64_t live_bytes_ = 0;
   std::uint64_t budget_;
 };
 
@@ -169,6 +176,9 @@ struct Context {
   std::vector<hipEvent_t> event_pool;
   std::uint64_t agg_launches = 0;
   std::uint64_t agg_bytes = 0;
+  // Cached views and the request loop (types private to the matcher).
+  std::shared_ptr<void> views;
+  std::shared_ptr<void> server;
 
   explicit Context(std::uint64_t budget) : arena(budget) {}
   ~Context() {

@@ -12,7 +12,7 @@ import re
 
 import numpy as np
 
-from ._abi import (R3DG_DEVICE, R3DG_HOST, STAGES, CImage, CMatchDebug, CMatchInfo, DisparityMap, Status, StereoOptions,
+from ._abi import (R3DG_DEVICE, R3DG_HOST, STAGES, CImage, CMatchDebug, CMatchInfo, CPair, DisparityMap, Status, StereoOptions,
                    align16)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -211,6 +211,96 @@ class Matcher:
                             int(info.tile_pixels), info.tiles_level0, labels, lr, info.seconds,
                             int(info.aggregate_launches), int(info.aggregate_bytes),
                             {k: float(info.stage_seconds[i]) for i, k in enumerate(STAGES)})
+
+    def match_batch(self, pairs, options=None, raise_errors=True):
+        """r3dg_match_batch: `pairs` is a sequence of dicts with left, right and
+        optional left_valid, right_valid, dmin (0), nd (64).  Returns one
+        DisparityMap per pair (None for a pair that failed when raise_errors
+        is False)."""
+        o = (options or StereoOptions()).to_c()
+        n = len(pairs)
+        arr = (CPair * max(n, 1))()
+        keep, outs = [], []
+        for i, q in enumerate(pairs):
+            left = q["left"]
+            h, w = _shape(left)
+            disp = _like(left, (h, w), np.float32)
+            conf = _like(left, (h, w), np.float32)
+            views = [_image(left, np.uint16), _image(q["right"], np.uint16), _image(q.get("left_valid"), np.uint8),
+                     _image(q.get("right_valid"), np.uint8), _image(disp, np.float32), _image(conf, np.float32)]
+            keep.append(views)
+            outs.append((disp, conf))
+            p = arr[i]
+            p.left, p.right, p.left_valid, p.right_valid = (v[0] for v in views[:4])
+            p.disparity, p.confidence = views[4][0], views[5][0]
+            p.disparity_min = int(q.get("dmin", 0))
+            p.num_disparities = int(q.get("nd", 64))
+        rc = lib().r3dg_match_batch(self._ctx, arr, n, ctypes.byref(o))
+        if rc != 0 and raise_errors:
+            self._check(rc)
+        res = []
+        for i in range(n):
+            info = arr[i].info
+            if arr[i].status != 0:
+                res.append(None)
+                continue
+            disp, conf = outs[i]
+            res.append(DisparityMap(disp, conf, info.disparity_min, info.num_disparities, info.levels,
+                                    int(info.cells), int(info.tile_pixels), info.tiles_level0, None, None,
+                                    info.seconds, int(info.aggregate_launches), int(info.aggregate_bytes),
+                                    {k: float(info.stage_seconds[j]) for j, k in enumerate(STAGES)}))
+        return res
+
+    # ---- serving: request loop, tickets, cached views
+    def serve_start(self, options=None, hbm_budget=0):
+        self._serve_opts = (options or StereoOptions()).to_c()
+        self._tickets = {}
+        self._check(lib().r3dg_server_start(self._ctx, ctypes.byref(self._serve_opts), ctypes.c_uint64(hbm_budget)))
+
+    def serve_stop(self):
+        self._check(lib().r3dg_server_stop(self._ctx))
+
+    def view_put(self, view_id, image, valid=None):
+        v, k1 = _image(image, np.uint16)
+        m, k2 = _image(valid, np.uint8)
+        self._check(lib().r3dg_view_put(self._ctx, ctypes.c_uint64(view_id), v, m))
+
+    def view_evict(self, view_id):
+        self._check(lib().r3dg_view_evict(self._ctx, ctypes.c_uint64(view_id)))
+
+    def submit(self, shape, dmin=0, nd=64, left=None, right=None, left_valid=None, right_valid=None, left_view=0,
+               right_view=0):
+        """Queues a pair; `shape` is (height, width) of the left image (the
+        output size).  Images not given are taken from the cached views."""
+        h, w = shape
+        disp = np.empty((h, w), np.float32)
+        conf = np.empty((h, w), np.float32)
+        views = [_image(left, np.uint16), _image(right, np.uint16), _image(left_valid, np.uint8),
+                 _image(right_valid, np.uint8), _image(disp, np.float32), _image(conf, np.float32)]
+        pair = CPair()
+        pair.left, pair.right, pair.left_valid, pair.right_valid = (v[0] for v in views[:4])
+        pair.disparity, pair.confidence = views[4][0], views[5][0]
+        pair.disparity_min = int(dmin)
+        pair.num_disparities = int(nd)
+        t = ctypes.c_uint64()
+        self._check(lib().r3dg_submit(self._ctx, ctypes.byref(pair), ctypes.c_uint64(left_view),
+                                      ctypes.c_uint64(right_view), ctypes.byref(t)))
+        self._tickets[t.value] = (pair, views, disp, conf, left, right, left_valid, right_valid)
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0):
+        """DisparityMap of a finished pair (info.seconds: submission to completion),
+        None on timeout; raises R3dgError if the pair failed."""
+        rc = lib().r3dg_wait(self._ctx, ctypes.c_uint64(ticket), ctypes.c_double(timeout))
+        if rc == Status.PENDING:
+            return None
+        pair, views, disp, conf = self._tickets.pop(ticket)[:4]
+        self._check(rc)
+        info = pair.info
+        return DisparityMap(disp, conf, info.disparity_min, info.num_disparities, info.levels, int(info.cells),
+                            int(info.tile_pixels), info.tiles_level0, None, None, info.seconds,
+                            int(info.aggregate_launches), int(info.aggregate_bytes),
+                            {k: float(info.stage_seconds[j]) for j, k in enumerate(STAGES)})
 
     def set_profiling(self, enabled=True):
         self._check(lib().r3dg_set_profiling(self._ctx, int(bool(enabled))))

@@ -16,6 +16,7 @@
 #include "../../oracle/oracle.hpp"
 #include "r3d/core/pyramid.hpp"
 #include "r3d/stereo/census.hpp"
+#include "r3d/stereo/stereo_batch.hpp"
 #include "r3d/stereo/stereo_matcher.hpp"
 
 namespace {
@@ -106,6 +107,13 @@ oracle::Img<T> to_oracle(const r3d::Image<T>& img) {
 
 bool same_bits(const r3d::ImageF32& a, const oracle::ImgF& b) {
   if (a.width() != b.w || a.height() != b.h) return false;
+  for (int y = 0; y < a.height(); ++y)
+    if (std::memcmp(a.row(y), b.row(y), sizeof(float) * std::size_t(a.width())) != 0) return false;
+  return true;
+}
+
+bool same_bits(const r3d::ImageF32& a, const r3d::ImageF32& b) {
+  if (a.width() != b.width() || a.height() != b.height()) return false;
   for (int y = 0; y < a.height(); ++y)
     if (std::memcmp(a.row(y), b.row(y), sizeof(float) * std::size_t(a.width())) != 0) return false;
   return true;
@@ -220,6 +228,34 @@ TEST(slanted, "Stereo matching recovers a slanted plane and equals the oracle") 
     CHECK(same_bits(r.disparity, c.disparity));
     CHECK(same_bits(r.confidence, c.confidence));
     CHECK(r.cells == c.cells);
+  }
+}
+
+TEST(batch, "match_stereo_batch gives every pair the map of match_stereo") {
+  const Texture tex(23);
+  r3d::ImageU16 l1, r1, l2, r2;
+  r3d::ImageF32 gt;
+  render(tex, 200, 120, [](double x, double y) { return 12.3 + 0.04 * x + 0.02 * y; }, l1, r1, gt);
+  render(tex, 150, 90, [](double x, double) { return 5.0 + 0.03 * x; }, l2, r2, gt);
+  std::vector<r3d::StereoPair> pairs(3);
+  pairs[0].left = l1.view();
+  pairs[0].right = r1.view();
+  pairs[0].num_disparities = 32;
+  pairs[1].left = l2.view();
+  pairs[1].right = r2.view();
+  pairs[1].disparity_min = -4;
+  pairs[1].num_disparities = 24;
+  pairs[2] = pairs[0];
+  pairs[2].num_disparities = 48;
+  const auto got = r3d::match_stereo_batch(pairs);
+  CHECK(got.size() == 3);
+  for (std::size_t i = 0; i < pairs.size(); ++i) {
+    const auto& p = pairs[i];
+    const r3d::DisparityMap one =
+        r3d::match_stereo(p.left, p.right, p.left_valid, p.right_valid, p.disparity_min, p.num_disparities);
+    CHECK(same_bits(got[i].disparity, one.disparity));
+    CHECK(same_bits(got[i].confidence, one.confidence));
+    CHECK(got[i].cells == one.cells && got[i].levels == one.levels);
   }
 }
 

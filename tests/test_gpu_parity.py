@@ -216,6 +216,21 @@ def test_match_config0_synthetic(gpu):
         _match_both(gpu, p.left, p.right, None, None, p.dmin, p.nd, StereoOptions(levels=levels))
 
 
+def test_match_chunked_volume(gpu):
+    """A context whose memory budget holds one tile's volume at a time runs the
+    aggregation in chunks of tiles; the maps are the same as in one chunk."""
+    from regard3d_mvs_amd.native import Matcher
+    p = synth.config_pair(0)
+    o = StereoOptions(levels=0)
+    ref = gpu.match(p.left, p.right, None, None, p.dmin, p.nd, o, debug=True)
+    with Matcher(0, hbm_budget=64 << 20) as small:
+        got = small.match(p.left, p.right, None, None, p.dmin, p.nd, o, debug=True)
+    assert got.cells == ref.cells
+    for a, b in ((got.disparity, ref.disparity), (got.confidence, ref.confidence), (got.labels, ref.labels),
+                 (got.lr_mask, ref.lr_mask)):
+        same_bits(a, b)
+
+
 def test_match_hierarchical_wide_range(gpu):
     from test_oracle_matcher import hierarchical_scene
     left, right, _, _ = hierarchical_scene()
