@@ -69,15 +69,18 @@ constexpr int kScanPerWave = 64 / kLanesPerScan;
 void launch_cost(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* row_prefix, int ntiles,
                  std::uint64_t total_rows, const std::uint64_t* row_start, const std::uint64_t* pix, std::uint8_t* C,
                  hipStream_t s);
-// One SGM path over every tile (path index 0..7, see kernels_sgm.hip).
-// `scratch` holds path_scratch_elems(groups, max_num) uint16: per scanline
-// two rows for the path costs of pixels wider than the LDS rows.
-std::size_t path_scratch_elems(std::uint64_t groups, int max_num);
+// One SGM path over every tile (path index 0..7, see kernels_sgm.hip): a
+// persistent launch of at most `slots` waves (path_slots(): the waves that fit
+// on the device) taking scanline groups from `queue` (a zeroed counter).
+// `scratch` holds path_scratch_elems(slots, max_num) uint16: per wave and
+// scanline two rows for the path costs of pixels wider than the LDS rows.
+std::uint64_t path_slots();
+std::size_t path_scratch_elems(std::uint64_t slots, int max_num);
 int path_scratch_row(int max_num);
 void launch_sgm_path(int path, const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* group_prefix, int ntiles,
                      std::uint64_t groups, const std::uint64_t* row_start, const std::uint64_t* pix,
-                     const std::uint8_t* C, std::uint16_t* S, std::uint16_t* scratch, int max_num, bool accumulate,
-                     hipStream_t s);
+                     const std::uint8_t* C, std::uint16_t* S, std::uint16_t* scratch, std::uint64_t slots, int max_num,
+                     std::uint32_t* queue, bool accumulate, hipStream_t s);
 // Extra elements S and C hold past their last cell.
 constexpr std::size_t kSPad = 64;
 // WTA + sub-pixel + checks of every tile row into per-tile-pixel results.

@@ -328,25 +328,50 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
     const JobDesc* __restrict__ jobs, const TileDesc* __restrict__ tiles, const std::uint64_t* __restrict__ gprefix,
     int ntiles, std::uint64_t groups, const std::uint64_t* __restrict__ row_start,
     const std::uint64_t* __restrict__ pix, const std::uint8_t* __restrict__ C, std::uint16_t* __restrict__ S,
-    std::uint16_t* __restrict__ scratch, int scratch_row) {
+    std::uint16_t* __restrict__ scratch, int scratch_row, std::uint32_t* __restrict__ queue) {
   __shared__ __attribute__((aligned(16))) std::uint16_t lut[kWaves][256];
   __shared__ __attribute__((aligned(16))) std::uint32_t rows[kWaves][kScanPerWave][2][kRow / 2];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63, grp = lane / kLanesPerScan, l8 = lane % kLanesPerScan;
-  const std::uint64_t gid = std::uint64_t(blockIdx.x) * kWaves + wave;
-  const bool wave_active = gid < groups;
-  const int t = find_tile(gprefix, ntiles, wave_active ? gid : gprefix[ntiles] - 1);
-  const TileDesc T = tiles[t];
-  const JobDesc& J = jobs[T.job];
-  for (int k = lane; k < 256; k += 64) lut[wave][k] = J.lut[k];
-  if (l8 < kPad / 2) {  // lead sentinels of both rows
+  // Resident wave index: the wave keeps its LDS rows and its scratch rows for
+  // every scanline group it takes from the queue.
+  const std::uint64_t slot = std::uint64_t(blockIdx.x) * kWaves + wave;
+  if (l8 < kPad / 2) {  // lead sentinels of both rows (never overwritten)
     rows[wave][grp][0][l8] = 0xFFFFFFFFU;
     rows[wave][grp][1][l8] = 0xFFFFFFFFU;
   }
-  __syncthreads();
-  if (!wave_active) return;
+  __attribute__((address_space(1))) std::uint32_t* srow =
+      reinterpret_cast<__attribute__((address_space(1))) std::uint32_t*>(
+          gptr_mut(scratch + (slot * kScanPerWave + grp) * 2 * std::uint64_t(scratch_row)));
+  const int swords = scratch_row / 2;
+  if (l8 < kPad / 2) {
+    srow[l8] = 0xFFFFFFFFU;
+    srow[swords + l8] = 0xFFFFFFFFU;
+  }
+  int lut_job = -1;
+  for (;;) {
+  // Next scanline group from the queue (one atomic per wave).
+  std::uint32_t item = 0;
+  if (lane == 0) item = atomicAdd(queue, 1U);
+  item = static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(item), 0));
+  if (item >= groups) break;
+  const int t = find_tile(gprefix, ntiles, item);
+  const TileDesc T = tiles[t];
+  const JobDesc& J = jobs[T.job];
+  if (T.job != lut_job) {
+    for (int k = lane; k < 256; k += 64) lut[wave][k] = J.lut[k];
+    lut_job = T.job;
+  }
+  __builtin_amdgcn_wave_barrier();
   const int ns = scanlines<DX, DY>(T.w, T.h);
-  const int s0 = static_cast<int>(gid - gprefix[t]) * kScanPerWave;
+  // Diagonal scanlines are longest in the middle of a tile: take those
+  // groups first so that a launch ends with short ones.
+  int rank = static_cast<int>(item - gprefix[t]);
+  if constexpr (DX != 0 && DY != 0) {
+    const int G = static_cast<int>(gprefix[t + 1] - gprefix[t]), mid = G / 2;
+    rank = (rank & 1) ? mid - 1 - (rank >> 1) : mid + (rank >> 1);
+  }
+  const int s0 = rank * kScanPerWave;
   const int sl = min(s0 + grp, ns - 1);  // idle groups shadow the last scanline
   const bool grp_active = s0 + grp < ns;
   int x0, y0, len;
@@ -366,17 +391,6 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
   const int img_stride = static_cast<int>(J.img_stride);
   const std::uint8_t* Ct = C + T.cell_base;
   std::uint16_t* St = S + T.cell_base;
-  // This scanline's two scratch rows (same layout as the LDS rows).
-  __attribute__((address_space(1))) std::uint32_t* srow =
-      scratch != nullptr
-          ? reinterpret_cast<__attribute__((address_space(1))) std::uint32_t*>(
-                gptr_mut(scratch + (gid * kScanPerWave + grp) * 2 * std::uint64_t(scratch_row)))
-          : nullptr;
-  const int swords = scratch_row / 2;
-  if (srow != nullptr && l8 < kPad / 2) {
-    srow[l8] = 0xFFFFFFFFU;
-    srow[swords + l8] = 0xFFFFFFFFU;
-  }
 
   auto load_in = [&](int i, StepIn& u) {
     i = min(i, len - 1);
@@ -623,6 +637,9 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
       (body(std::integral_constant<int, K>{}, i + K), ...);
     }(std::make_integer_sequence<int, NU * NV>{});
   }
+  // Retire this group's loads before the next group reuses their registers.
+  __builtin_amdgcn_s_waitcnt(kVmcnt0);
+  }
 }
 
 // ------------------------------------------------------------------ selection
@@ -838,17 +855,37 @@ __global__ void k_pack_rows(const std::int32_t* __restrict__ base, const std::in
   reinterpret_cast<int4*>(rows4)[y] = make_int4(base[2 * y], base[2 * y + 1], partner[2 * y], partner[2 * y + 1]);
 }
 
+// Blocks of a kernel that fit on the device at once.
+template <typename K>
+std::uint64_t resident_blocks(K kernel) {
+  int dev = 0, cus = 0, per_cu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 64 * kWaves, 0);
+  return std::uint64_t(std::max(cus, 1)) * std::uint64_t(std::max(per_cu, 1));
+}
+
+template <int DX, int DY, bool kAcc>
+std::uint64_t path_blocks() {
+  static const std::uint64_t n = resident_blocks(k_path<DX, DY, kAcc>);
+  return n;
+}
+
 template <int DX, int DY>
 void path_launch(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* gp, int ntiles,
                  std::uint64_t groups, const std::uint64_t* rs, const std::uint64_t* pix, const std::uint8_t* C,
-                 std::uint16_t* S, std::uint16_t* scratch, int scratch_row, bool accumulate, hipStream_t s) {
-  const std::uint64_t blocks = (groups + kWaves - 1) / kWaves;
+                 std::uint16_t* S, std::uint16_t* scratch, int scratch_row, std::uint32_t* queue, std::uint64_t slots,
+                 bool accumulate, hipStream_t s) {
+  // Persistent waves: at most `slots` (scratch rows), at most one per group.
+  const std::uint64_t waves = std::min(groups, slots);
   if (accumulate) {
+    const std::uint64_t blocks = std::min((waves + kWaves - 1) / kWaves, path_blocks<DX, DY, true>());
     k_path<DX, DY, true><<<dim3(static_cast<unsigned>(blocks)), 64 * kWaves, 0, s>>>(
-        jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, scratch_row);
+        jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, scratch_row, queue);
   } else {
+    const std::uint64_t blocks = std::min((waves + kWaves - 1) / kWaves, path_blocks<DX, DY, false>());
     k_path<DX, DY, false><<<dim3(static_cast<unsigned>(blocks)), 64 * kWaves, 0, s>>>(
-        jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, scratch_row);
+        jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, scratch_row, queue);
   }
 }
 
@@ -875,24 +912,34 @@ void launch_cost(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t
 
 int path_scratch_row(int max_num) { return kPad + (max_num + 15) / 16 * 16 + kPad; }
 
-std::size_t path_scratch_elems(std::uint64_t groups, int max_num) {
-  return std::size_t(groups) * kScanPerWave * 2 * std::size_t(path_scratch_row(max_num));
+std::uint64_t path_slots() {
+  std::uint64_t b = path_blocks<1, 0, true>();
+  b = std::max(b, path_blocks<1, 0, false>());
+  b = std::max(b, path_blocks<0, 1, true>());
+  b = std::max(b, path_blocks<1, 1, true>());
+  b = std::max(b, path_blocks<-1, 1, true>());
+  return b * kWaves;
+}
+
+std::size_t path_scratch_elems(std::uint64_t slots, int max_num) {
+  return std::size_t(slots) * kScanPerWave * 2 * std::size_t(path_scratch_row(max_num));
 }
 
 void launch_sgm_path(int path, const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* gp, int ntiles,
                      std::uint64_t groups, const std::uint64_t* rs, const std::uint64_t* pix, const std::uint8_t* C,
-                     std::uint16_t* S, std::uint16_t* scratch, int max_num, bool accumulate, hipStream_t s) {
+                     std::uint16_t* S, std::uint16_t* scratch, std::uint64_t slots, int max_num,
+                     std::uint32_t* queue, bool accumulate, hipStream_t s) {
   if (groups == 0) return;
   const int srow = path_scratch_row(max_num);
   switch (path) {
-    case 0: path_launch<1, 0>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, accumulate, s); break;
-    case 1: path_launch<-1, 0>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, accumulate, s); break;
-    case 2: path_launch<0, 1>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, accumulate, s); break;
-    case 3: path_launch<0, -1>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, accumulate, s); break;
-    case 4: path_launch<1, 1>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, accumulate, s); break;
-    case 5: path_launch<-1, -1>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, accumulate, s); break;
-    case 6: path_launch<-1, 1>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, accumulate, s); break;
-    default: path_launch<1, -1>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, accumulate, s); break;
+    case 0: path_launch<1, 0>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, queue, slots, accumulate, s); break;
+    case 1: path_launch<-1, 0>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, queue, slots, accumulate, s); break;
+    case 2: path_launch<0, 1>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, queue, slots, accumulate, s); break;
+    case 3: path_launch<0, -1>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, queue, slots, accumulate, s); break;
+    case 4: path_launch<1, 1>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, queue, slots, accumulate, s); break;
+    case 5: path_launch<-1, -1>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, queue, slots, accumulate, s); break;
+    case 6: path_launch<-1, 1>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, queue, slots, accumulate, s); break;
+    default: path_launch<1, -1>(jobs, tiles, gp, ntiles, groups, rs, pix, C, S, scratch, srow, queue, slots, accumulate, s); break;
   }
 }
 

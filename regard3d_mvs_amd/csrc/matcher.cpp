@@ -114,8 +114,17 @@ DImg<T> input_img(Context& c, const r3dg_image& v, const char* name) {
     return d;
   }
   DImg<T> d = alloc_img<T>(c, v.width, v.height);
-  hip_check(hipMemcpy2DAsync(d.p, std::size_t(d.s) * sizeof(T), v.data, std::size_t(v.stride) * sizeof(T),
-                             std::size_t(v.width) * sizeof(T), std::size_t(v.height), hipMemcpyHostToDevice, c.stream),
+  const std::size_t row = std::size_t(v.width) * sizeof(T), bytes = row * std::size_t(v.height);
+  void* pin = bytes >= (std::size_t{1} << 20) ? c.pinned.get(bytes) : nullptr;
+  if (pin != nullptr) {
+    // Stage through pinned memory (released at the next stream sync).
+    c.staged_blocks.push_back(pin);
+    copy_rows(pin, row, v.data, std::size_t(v.stride) * sizeof(T), row, std::size_t(v.height));
+    hip_check(hipMemcpyAsync(d.p, pin, bytes, hipMemcpyHostToDevice, c.stream), "upload");
+    return d;
+  }
+  hip_check(hipMemcpy2DAsync(d.p, std::size_t(d.s) * sizeof(T), v.data, std::size_t(v.stride) * sizeof(T), row,
+                             std::size_t(v.height), hipMemcpyHostToDevice, c.stream),
             "upload");
   return d;
 }
@@ -128,25 +137,43 @@ void output_img(Context& c, const r3dg_image& v, const T* src, int w, int h, std
                                           std::to_string(v.height) + " differs from " + std::to_string(w) + "x" +
                                           std::to_string(h));
   if (v.stride < v.width) fail(R3DG_ERROR_INVALID_ARGUMENT, std::string(name) + ": stride smaller than width");
+  const std::size_t row = std::size_t(w) * sizeof(T), bytes = row * std::size_t(h);
+  if (v.memory != R3DG_DEVICE && bytes >= (std::size_t{1} << 20)) {
+    if (void* pin = c.pinned.get(bytes)) {
+      // Into pinned memory; copied to the caller's buffer at the next sync.
+      c.staged_blocks.push_back(pin);
+      hip_check(hipMemcpy2DAsync(pin, row, src, std::size_t(ss) * sizeof(T), row, std::size_t(h),
+                                 hipMemcpyDeviceToHost, c.stream),
+                "download");
+      c.staged_out.push_back({pin, v.data, row, std::size_t(h), std::size_t(v.stride) * sizeof(T)});
+      return;
+    }
+  }
   const hipMemcpyKind kind = v.memory == R3DG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-  hip_check(hipMemcpy2DAsync(v.data, std::size_t(v.stride) * sizeof(T), src, std::size_t(ss) * sizeof(T),
-                             std::size_t(w) * sizeof(T), std::size_t(h), kind, c.stream),
+  hip_check(hipMemcpy2DAsync(v.data, std::size_t(v.stride) * sizeof(T), src, std::size_t(ss) * sizeof(T), row,
+                             std::size_t(h), kind, c.stream),
             "download");
 }
 
-// Small host-to-device copy on the context's stream.  Waits for it, so the
-// host vector may be released right after (pageable-memory copies are not
-// guaranteed to have consumed the source when the async call returns).
+// Small host-to-device copy on the context's stream, staged through a pinned
+// block (returned at the next sync), so the host vector may be released right
+// away and the host does not wait for the device.
 template <typename T>
 void upload(Context& c, DBuf<T>& dst, const std::vector<T>& src) {
   dst = DBuf<T>(c.arena, std::max<std::size_t>(src.size(), 1));
-  if (!src.empty()) {
-    hip_check(hipMemcpyAsync(dst.get(), src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, c.stream), "upload");
-    hip_check(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+  if (src.empty()) return;
+  const std::size_t bytes = src.size() * sizeof(T);
+  if (void* pin = c.pinned.get(bytes)) {
+    c.staged_blocks.push_back(pin);
+    std::memcpy(pin, src.data(), bytes);
+    hip_check(hipMemcpyAsync(dst.get(), pin, bytes, hipMemcpyHostToDevice, c.stream), "upload");
+    return;
   }
+  hip_check(hipMemcpyAsync(dst.get(), src.data(), bytes, hipMemcpyHostToDevice, c.stream), "upload");
+  hip_check(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
 }
 
-void sync(Context& c) { hip_check(hipStreamSynchronize(c.stream), "hipStreamSynchronize"); }
+void sync(Context& c) { c.sync(); }
 void check_launch() { hip_check(hipGetLastError(), "kernel launch"); }
 
 // ------------------------------------------------------------------ stages
@@ -557,8 +584,13 @@ Chunk aggregate_tiles(Context& c, Aggregation& A, int t0, int t1) {
   DBuf<std::uint64_t> dgp;
   upload(c, dgp, gp);
   ch.S = DBuf<std::uint16_t>(c.arena, ch.cells + kSPad);
-  // Scratch rows for pixels wider than the kernel's LDS rows.
-  DBuf<std::uint16_t> scratch(c.arena, std::max<std::size_t>(path_scratch_elems(max_groups, max_num), 1));
+  // Scratch rows of the persistent waves (pixels wider than the LDS rows) and
+  // the work-queue counters of the 8 launches.
+  // (a multiple of 4: launches use whole blocks of 4 waves, one slot each)
+  const std::uint64_t slots = (std::min(path_slots(), std::max<std::uint64_t>(max_groups, 1)) + 3) / 4 * 4;
+  DBuf<std::uint16_t> scratch(c.arena, path_scratch_elems(slots, max_num));
+  DBuf<std::uint32_t> queues(c.arena, 8);
+  hip_check(hipMemsetAsync(queues.get(), 0, 8 * sizeof(std::uint32_t), c.stream), "hipMemsetAsync");
 
   // Matching costs of every cell (timed with the other cost preparation).
   DBuf<std::uint8_t> C(c.arena, ch.cells + kSPad);
@@ -574,7 +606,8 @@ Chunk aggregate_tiles(Context& c, Aggregation& A, int t0, int t1) {
     for (int p = 0; p < 8; ++p) {
       const std::size_t o8 = std::size_t(p) * (std::size_t(nt) + 1);
       launch_sgm_path(p, A.djobs.get(), ch.dtiles.get(), dgp.get() + o8, nt, gp[o8 + std::size_t(nt)],
-                      A.row_start.get(), A.pix.get(), C.get(), ch.S.get(), scratch.get(), max_num, p > 0, c.stream);
+                      A.row_start.get(), A.pix.get(), C.get(), ch.S.get(), scratch.get(), slots, max_num,
+                      queues.get() + p, p > 0, c.stream);
       check_launch();
     }
   }
@@ -589,8 +622,10 @@ Chunk aggregate_tiles(Context& c, Aggregation& A, int t0, int t1) {
 // in the context's budget, one chunk when everything fits.
 std::vector<std::array<int, 2>> chunk_tiles(const Context& c, const Aggregation& A) {
   const std::uint64_t free_bytes = c.arena.budget() > c.arena.live() ? c.arena.budget() - c.arena.live() : 0;
-  // 3 bytes per cell (S and C) with room for scratch rows and per-tile data.
-  const std::uint64_t cap = std::max(A.max_tile_cells, free_bytes / 4);
+  // 3 bytes per cell (S and C) with room for scratch rows and per-tile data;
+  // at most 4 G cells, enough to fill the device, so that the volume blocks
+  // are reused from chunk to chunk rather than freed and mapped again.
+  const std::uint64_t cap = std::max(A.max_tile_cells, std::min(free_bytes / 4, std::uint64_t{4} << 30));
   std::vector<std::array<int, 2>> out;
   const int nt = static_cast<int>(A.td.size());
   int t0 = 0;
@@ -964,9 +999,11 @@ int32_t guarded(Context* c, F&& f) {
   if (c != nullptr) {
     c->error = msg;
     c->error_status = status;
-    // Leave the stream idle so the context stays usable.
+    // Leave the stream idle so the context stays usable; staged outputs of
+    // the failed call are dropped.
     (void)hipStreamSynchronize(c->stream);
     (void)hipGetLastError();
+    c->drop_staged();
   }
   g_error = msg;
   g_status = status;
@@ -1051,6 +1088,25 @@ class Server {
     if (worker_.joinable()) worker_.join();
   }
 
+  // A view put (host copies of the image and mask) or evict, applied by the
+  // loop before it admits the requests submitted after it.
+  struct ViewOp {
+    std::uint64_t id = 0;
+    bool evict = false;
+    int w = 0, h = 0, mw = 0, mh = 0;
+    std::vector<std::uint16_t> img;
+    std::vector<std::uint8_t> mask;
+  };
+
+  void view_op(ViewOp op) {
+    {
+      const std::lock_guard<std::mutex> lock(m_);
+      if (stop_) fail(R3DG_ERROR_INVALID_ARGUMENT, "serve: the request loop is stopping");
+      view_ops_.push_back(std::move(op));
+    }
+    cv_.notify_all();
+  }
+
   std::uint64_t submit(r3dg_pair* pair, std::uint64_t lv, std::uint64_t rv) {
     auto r = std::make_unique<Request>();
     r->pair = pair;
@@ -1121,10 +1177,12 @@ class Server {
     std::vector<Request*> active;
     for (;;) {
       std::vector<Request*> admit;
+      std::deque<ViewOp> ops;
       {
         std::unique_lock<std::mutex> lock(m_);
-        cv_.wait(lock, [&] { return stop_ || !pending_.empty() || !active.empty(); });
-        if (stop_ && pending_.empty() && active.empty()) return;
+        cv_.wait(lock, [&] { return stop_ || !pending_.empty() || !active.empty() || !view_ops_.empty(); });
+        if (stop_ && pending_.empty() && active.empty() && view_ops_.empty()) return;
+        ops.swap(view_ops_);
         // Admission by estimated footprint; a pair that fits nowhere runs alone.
         while (!pending_.empty()) {
           Request* r = reqs_.at(pending_.front()).get();
@@ -1136,6 +1194,26 @@ class Server {
       }
       const std::lock_guard<std::mutex> ctx_lock(c_.mutex);
       (void)hipSetDevice(c_.device);
+      for (ViewOp& op : ops) {
+        (void)guarded(&c_, [&] {
+          ViewCache& vc = view_cache(c_);
+          if (op.evict) {
+            vc.erase(op.id);
+            return;
+          }
+          auto v = std::make_shared<View>();
+          v->img = alloc_img<std::uint16_t>(c_, op.w, op.h);
+          hip_check(hipMemcpyAsync(v->img.p, op.img.data(), op.img.size() * 2, hipMemcpyHostToDevice, c_.stream),
+                    "upload");
+          if (!op.mask.empty()) {
+            v->mask = alloc_img<std::uint8_t>(c_, op.mw, op.mh);
+            hip_check(hipMemcpyAsync(v->mask.p, op.mask.data(), op.mask.size(), hipMemcpyHostToDevice, c_.stream),
+                      "upload");
+          }
+          sync(c_);
+          vc[op.id] = std::move(v);
+        });
+      }
       for (Request* r : admit) {
         const int32_t st = guarded(&c_, [&] {
           r3dg_pair& q = *r->pair;
@@ -1197,6 +1275,7 @@ class Server {
   std::mutex m_;
   std::condition_variable cv_, done_cv_;
   std::deque<std::uint64_t> pending_;
+  std::deque<ViewOp> view_ops_;
   std::unordered_map<std::uint64_t, std::unique_ptr<Request>> reqs_;
   std::uint64_t next_ = 1, in_use_ = 0;
   bool stop_ = false;
@@ -1506,19 +1585,22 @@ int32_t r3dg_match_batch(r3dg_ctx* h, r3dg_pair* pairs, int32_t count, const r3d
     c.collect(stage_seconds, R3DG_STAGES);  // drop marks of earlier calls
     std::fill(stage_seconds, stage_seconds + R3DG_STAGES, 0.0);
     // Groups whose per-pixel buffers (pyramids, Census, maps: about 96 bytes
-    // per input pixel) take at most half of the memory left; the volumes are
-    // chunked to the rest by match_tiles.
+    // per input pixel) take at most half of the memory left, and of at most
+    // 256 M left pixels (enough to fill the device at every level but the
+    // coarsest); the volumes are chunked to the rest by match_tiles.
     const std::uint64_t room = c.arena.budget() > c.arena.live() ? (c.arena.budget() - c.arena.live()) / 2 : 0;
     std::size_t g0 = 0;
     while (g0 < ok.size()) {
       std::size_t g1 = g0;
-      std::uint64_t need = 0;
+      std::uint64_t need = 0, px = 0;
       while (g1 < ok.size()) {
         const r3dg_pair& q = pairs[ok[g1]];
         const std::uint64_t est = 96 * (std::uint64_t(q.left.width) + std::uint64_t(q.right.width)) *
                                   std::uint64_t(q.left.height);
-        if (g1 > g0 && need + est > room) break;
+        const std::uint64_t p = std::uint64_t(q.left.width) * std::uint64_t(q.left.height);
+        if (g1 > g0 && (need + est > room || px + p > (std::uint64_t{256} << 20))) break;
         need += est;
+        px += p;
         ++g1;
       }
       std::vector<PairRun*> group;
@@ -1564,18 +1646,27 @@ int32_t r3dg_server_start(r3dg_ctx* h, const r3dg_options* options, uint64_t hbm
     r3dg_options tmp;
     const r3dg_options& o = opts_or_default(options, tmp);
     check_options(o);
+    const std::lock_guard<std::mutex> lock(c.server_mutex);
     if (c.server) fail(R3DG_ERROR_INVALID_ARGUMENT, "serve: the request loop is already running");
     const std::uint64_t budget = hbm_budget != 0 ? hbm_budget : c.arena.budget() / 10 * 8;
     c.server = std::make_shared<Server>(c, o, budget);
   });
 }
 
+namespace {
+// The context's request loop, or nullptr (without waiting for a loop step).
+std::shared_ptr<Server> running_server(Context& c) {
+  const std::lock_guard<std::mutex> lock(c.server_mutex);
+  return std::static_pointer_cast<Server>(c.server);
+}
+}  // namespace
+
 int32_t r3dg_server_stop(r3dg_ctx* h) {
   if (h == nullptr) return with_ctx(h, [](Context&) {});
   auto* c = reinterpret_cast<Context*>(h);
   std::shared_ptr<void> srv;
   {
-    const std::lock_guard<std::mutex> lock(c->mutex);
+    const std::lock_guard<std::mutex> lock(c->server_mutex);
     srv = std::move(c->server);
   }
   // The loop takes the context lock for its steps: stop it without holding it.
@@ -1587,13 +1678,9 @@ int32_t r3dg_submit(r3dg_ctx* h, r3dg_pair* pair, uint64_t left_view, uint64_t r
   if (h == nullptr || pair == nullptr || ticket == nullptr) return with_ctx(nullptr, [](Context&) {});
   auto* c = reinterpret_cast<Context*>(h);
   return guarded(nullptr, [&] {
-    std::shared_ptr<void> srv;
-    {
-      const std::lock_guard<std::mutex> lock(c->mutex);
-      srv = c->server;
-    }
+    const std::shared_ptr<Server> srv = running_server(*c);
     if (!srv) fail(R3DG_ERROR_INVALID_ARGUMENT, "serve: no request loop (r3dg_server_start)");
-    *ticket = static_cast<Server*>(srv.get())->submit(pair, left_view, right_view);
+    *ticket = srv->submit(pair, left_view, right_view);
   });
 }
 
@@ -1603,13 +1690,9 @@ int32_t wait_ticket(r3dg_ctx* h, uint64_t ticket, double timeout) {
   auto* c = reinterpret_cast<Context*>(h);
   int32_t st = R3DG_OK;
   const int32_t rc = guarded(nullptr, [&] {
-    std::shared_ptr<void> srv;
-    {
-      const std::lock_guard<std::mutex> lock(c->mutex);
-      srv = c->server;
-    }
+    const std::shared_ptr<Server> srv = running_server(*c);
     if (!srv) fail(R3DG_ERROR_INVALID_ARGUMENT, "serve: no request loop (r3dg_server_start)");
-    st = static_cast<Server*>(srv.get())->wait(ticket, timeout);
+    st = srv->wait(ticket, timeout);
   });
   return rc != R3DG_OK ? rc : st;
 }
@@ -1622,6 +1705,38 @@ int32_t r3dg_wait(r3dg_ctx* h, uint64_t ticket, double timeout_seconds) {
 }
 
 int32_t r3dg_view_put(r3dg_ctx* h, uint64_t view, r3dg_image image, r3dg_image valid) {
+  if (h != nullptr && image.memory == R3DG_HOST && (view_empty(valid) || valid.memory == R3DG_HOST)) {
+    auto* c = reinterpret_cast<Context*>(h);
+    if (std::shared_ptr<Server> srv = running_server(*c)) {
+      // Host images while the loop runs: copy them and let the loop upload
+      // them between its steps, so the caller does not wait for a step.
+      return guarded(nullptr, [&] {
+        if (view == 0) fail(R3DG_ERROR_INVALID_ARGUMENT, "view: id 0 is reserved");
+        if (view_empty(image) || image.stride < image.width) fail(R3DG_ERROR_INVALID_ARGUMENT, "view: bad image");
+        Server::ViewOp op;
+        op.id = view;
+        op.w = image.width;
+        op.h = image.height;
+        op.img.resize(std::size_t(image.width) * std::size_t(image.height));
+        for (int y = 0; y < image.height; ++y)
+          std::memcpy(op.img.data() + std::size_t(y) * std::size_t(image.width),
+                      static_cast<const std::uint16_t*>(image.data) + std::size_t(y) * std::size_t(image.stride),
+                      std::size_t(image.width) * 2);
+        if (!view_empty(valid)) {
+          if (valid.width != image.width || valid.height != image.height || valid.stride < valid.width)
+            fail(R3DG_ERROR_INVALID_ARGUMENT, "view: mask size differs from the image");
+          op.mw = valid.width;
+          op.mh = valid.height;
+          op.mask.resize(std::size_t(valid.width) * std::size_t(valid.height));
+          for (int y = 0; y < valid.height; ++y)
+            std::memcpy(op.mask.data() + std::size_t(y) * std::size_t(valid.width),
+                        static_cast<const std::uint8_t*>(valid.data) + std::size_t(y) * std::size_t(valid.stride),
+                        std::size_t(valid.width));
+        }
+        srv->view_op(std::move(op));
+      });
+    }
+  }
   return with_ctx(h, [&](Context& c) {
     if (view == 0) fail(R3DG_ERROR_INVALID_ARGUMENT, "view: id 0 is reserved");
     if (view_empty(image)) fail(R3DG_ERROR_INVALID_ARGUMENT, "view: empty image");
@@ -1651,6 +1766,18 @@ int32_t r3dg_view_put(r3dg_ctx* h, uint64_t view, r3dg_image image, r3dg_image v
 }
 
 int32_t r3dg_view_evict(r3dg_ctx* h, uint64_t view) {
+  if (h != nullptr) {
+    auto* c = reinterpret_cast<Context*>(h);
+    if (std::shared_ptr<Server> srv = running_server(*c)) {
+      // In order with the puts queued before it.
+      return guarded(nullptr, [&] {
+        Server::ViewOp op;
+        op.id = view;
+        op.evict = true;
+        srv->view_op(std::move(op));
+      });
+    }
+  }
   return with_ctx(h, [&](Context& c) {
     if (view_cache(c).erase(view) == 0) fail(R3DG_ERROR_NOT_FOUND, "view: unknown id " + std::to_string(view));
   });

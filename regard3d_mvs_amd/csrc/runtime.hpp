@@ -6,6 +6,9 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdint>
+#include <thread>
+#include <cstring>
+#include <iterator>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -57,7 +60,7 @@ class Arena {
       free_.erase(it);
       return p;
     }
-    if (held_ + size > budget_) trim();
+    if (held_ + size > budget_) release_cached(held_ + size - budget_);
     if (held_ + size > budget_) {
       fail(R3DG_ERROR_BUDGET, "device memory budget exceeded: need " + std::to_string(size >> 20) + " MiB more, " +
                                   std::to_string(held_ >> 20) + " of " + std::to_string(budget_ >> 20) +
@@ -94,6 +97,19 @@ class Arena {
     free_.clear();
   }
 
+  // Frees cached blocks, largest first, until `bytes` are returned (or none
+  // is left); smaller blocks stay cached for the shapes that reuse them.
+  void release_cached(std::uint64_t bytes) {
+    std::uint64_t done = 0;
+    while (done < bytes && !free_.empty()) {
+      auto it = std::prev(free_.end());
+      (void)hipFree(it->second);
+      held_ -= it->first;
+      done += it->first;
+      free_.erase(it);
+    }
+  }
+
   [[nodiscard]] std::uint64_t held() const { return held_; }
   // Bytes of blocks in use (held minus the cached free blocks).
   [[nodiscard]] std::uint64_t live() const { return live_bytes_; }
@@ -110,6 +126,72 @@ class Arena {
   std::uint64_t live_bytes_ = 0;
   std::uint64_t budget_;
 };
+
+// Cached pinned host blocks (hipHostMalloc), for staging transfers of host
+// images: pinned copies run at DMA speed, pageable strided ones do not.
+class PinnedPool {
+ public:
+  PinnedPool() = default;
+  PinnedPool(const PinnedPool&) = delete;
+  PinnedPool& operator=(const PinnedPool&) = delete;
+  ~PinnedPool() {
+    for (auto& [size, p] : free_) (void)hipHostFree(p);
+  }
+  void* get(std::size_t bytes) {
+    const std::size_t g = bytes >= (std::size_t{1} << 20) ? (std::size_t{1} << 20) : (std::size_t{64} << 10);
+    const std::size_t size = (std::max<std::size_t>(bytes, 1) + g - 1) / g * g;
+    auto it = free_.lower_bound(size);
+    if (it != free_.end() && it->first <= 2 * size) {
+      void* p = it->second;
+      sizes_[p] = it->first;
+      free_.erase(it);
+      return p;
+    }
+    void* p = nullptr;
+    if (hipHostMalloc(&p, size, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      for (auto& [sz, q] : free_) (void)hipHostFree(q);
+      free_.clear();
+      if (hipHostMalloc(&p, size, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+      }
+    }
+    sizes_[p] = size;
+    return p;
+  }
+  void put(void* p) {
+    auto it = sizes_.find(p);
+    if (it == sizes_.end()) return;
+    free_.emplace(it->second, p);
+    sizes_.erase(it);
+  }
+
+ private:
+  std::multimap<std::size_t, void*> free_;
+  std::unordered_map<void*, std::size_t> sizes_;
+};
+
+// Copies `rows` rows of `row_bytes` between strided buffers, split over a few
+// threads when large.
+inline void copy_rows(void* dst, std::size_t dst_stride, const void* src, std::size_t src_stride,
+                      std::size_t row_bytes, std::size_t rows) {
+  auto part = [&](std::size_t r0, std::size_t r1) {
+    for (std::size_t r = r0; r < r1; ++r)
+      std::memcpy(static_cast<char*>(dst) + r * dst_stride, static_cast<const char*>(src) + r * src_stride, row_bytes);
+  };
+  const std::size_t total = row_bytes * rows;
+  const std::size_t nt = std::min<std::size_t>(8, std::max<std::size_t>(1, total >> 23));
+  if (nt <= 1 || rows < nt) {
+    part(0, rows);
+    return;
+  }
+  std::vector<std::thread> th;
+  const std::size_t per = (rows + nt - 1) / nt;
+  for (std::size_t k = 1; k < nt; ++k) th.emplace_back(part, std::min(rows, k * per), std::min(rows, (k + 1) * per));
+  part(0, std::min(rows, per));
+  for (auto& t : th) t.join();
+}
 
 // RAII device buffer of n elements of T from an arena.
 template <typename T>
@@ -176,8 +258,37 @@ struct Context {
   std::vector<hipEvent_t> event_pool;
   std::uint64_t agg_launches = 0;
   std::uint64_t agg_bytes = 0;
-  // Cached views and the request loop (types private to the matcher).
+  // Host transfers staged through pinned blocks: outputs are copied to the
+  // caller's buffers, and the blocks returned, when the stream is next synced.
+  PinnedPool pinned;
+  struct StagedOut {
+    void* pinned;
+    void* dst;
+    std::size_t row_bytes, rows, dst_stride;
+  };
+  std::vector<StagedOut> staged_out;
+  std::vector<void*> staged_blocks;
+  // Waits for the stream, then completes the staged outputs.
+  void sync() {
+    const hipError_t e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) {
+      for (const StagedOut& o : staged_out) copy_rows(o.dst, o.dst_stride, o.pinned, o.row_bytes, o.row_bytes, o.rows);
+    }
+    staged_out.clear();
+    for (void* p : staged_blocks) pinned.put(p);
+    staged_blocks.clear();
+    hip_check(e, "hipStreamSynchronize");
+  }
+  // After a failure: drop staged outputs without copying them.
+  void drop_staged() {
+    staged_out.clear();
+    for (void* p : staged_blocks) pinned.put(p);
+    staged_blocks.clear();
+  }
+  // Cached views and the request loop (types private to the matcher).  The
+  // loop pointer has its own lock: submitting must not wait for a step.
   std::shared_ptr<void> views;
+  std::mutex server_mutex;
   std::shared_ptr<void> server;
 
   explicit Context(std::uint64_t budget) : arena(budget) {}
