@@ -425,9 +425,11 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
       d.s[3] = v.w;
     }
   };
+  // Segment 1 only when a scanline of the wave needs it at that step (its
+  // pixel has more than 32 labels); the step tests the same condition.
   auto load_segs = [&](const StepIn& u, StepSegs& d) {
     load_seg4(u, d.a);
-    load_seg8(u, 1, d.b);
+    if (__builtin_amdgcn_ballot_w64(grp_active && pix_num(u.rec) > 32) != 0) load_seg8(u, 1, d.b);
   };
 
   int prev_lo = 0, prev_num = 0, flip = 0;
