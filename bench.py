@@ -227,11 +227,12 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=2, choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-rows", type=int, default=2016, help="rows of the CPU-baseline band (0: skip)")
+    ap.add_argument("--cpu-rows", type=int, default=4032, help="rows of the CPU-baseline band (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--workload", default="pair", choices=["pair", "batch", "serve"],
+    ap.add_argument("--workload", default="pair", choices=["pair", "batch", "serve", "copy"],
                     help="pair: one pair per call (default); batch: the 160-pair scene, batched vs plain; "
-                         "serve: a Poisson request stream through the serving loop vs plain")
+                         "serve: a Poisson request stream through the serving loop vs plain; copy: one 1 GiB "
+                         "device-to-device copy (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--views", type=int, default=40, help="batch: views (4 partners each)")
     ap.add_argument("--requests", type=int, default=300, help="serve: requests per rate")
     ap.add_argument("--rates", default="5,20,50", help="serve: arrival rates (pairs/s)")
@@ -241,6 +242,17 @@ def main() -> int:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(pairs are independent, see DESIGN.md for how they would shard)", file=sys.stderr)
         return 2
+    if args.workload == "copy":
+        import torch
+        n = 1 << 30
+        a = torch.empty(n, dtype=torch.uint8, device="cuda")
+        a.fill_(1)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copied_bytes": n}))
+        return 0
     if args.workload == "batch":
         return bench_batch(args)
     if args.workload == "serve":
