@@ -191,6 +191,8 @@ __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, 
                                               const std::uint64_t* __restrict__ row_prefix, int ntiles,
                                               const std::uint64_t* __restrict__ row_start,
                                               const std::uint64_t* __restrict__ pix, std::uint8_t* __restrict__ C) {
+  constexpr int kStage = 4096;  // row pixels whose cell offsets are staged in LDS
+  __shared__ std::uint32_t soff[kStage];
   const std::uint64_t r = blockIdx.x;
   const int t = find_tile(row_prefix, ntiles, r);
   const TileDesc T = tiles[t];
@@ -198,6 +200,12 @@ __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, 
   const int y = static_cast<int>(r - row_prefix[t]);
   const int Y = T.y0 + y;
   const auto rec = gptr(pix + T.px_base + std::uint64_t(y) * T.w);
+  const bool staged = T.w <= kStage;
+  if (staged) {
+    for (int x = threadIdx.x; x < T.w; x += blockDim.x) soff[x] = pix_off(rec[x]);
+    __syncthreads();
+  }
+  auto off_at = [&](int x) { return staged ? soff[x] : pix_off(rec[x]); };
   const std::uint64_t last = rec[T.w - 1];
   const std::uint32_t chunks = (pix_off(last) + static_cast<std::uint32_t>(pix_num(last))) >> 4;
   const auto rows = gptr(J.rows4) + 4 * Y;
@@ -212,7 +220,7 @@ __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, 
     int xa = 0, xb = min(static_cast<int>(k), T.w - 1);
     while (xa < xb) {
       const int mid = (xa + xb + 1) >> 1;
-      if (pix_off(rec[mid]) <= 16 * k) {
+      if (off_at(mid) <= 16 * k) {
         xa = mid;
       } else {
         xb = mid - 1;
@@ -278,7 +286,7 @@ __device__ __forceinline__ void stb(void* base, std::uint32_t off, T v) {
 constexpr int kVmcnt0 = 0x0F70;
 constexpr int kWaves = 4;                     // waves per block
 constexpr int kDist = 1;                      // steps between a segment prefetch and its use
-constexpr int kNarrow = 96;                   // labels whose path costs stay in LDS
+constexpr int kNarrow = 256;                  // labels whose path costs stay in LDS
 constexpr int kPad = 12;                      // sentinel slots before label 0 and after the last label
 constexpr int kRow = kPad + kNarrow + kPad;   // uint16 slots of an LDS row
 
