@@ -53,21 +53,23 @@ void launch_ranges_from_coarse(const float* coarse, int wc, int hc, int width, i
 // ---- aggregation (kernels_sgm.hip)
 // Cells of every tile row (row_cells[row_base + y]) and the pixel records
 // (cell offset inside the row, lo, num); one block per tile row.
-// row_max = largest label count of the row.
+// row_max = largest label count of the row; bstart[band_base + b * h + y] =
+// row offset of band b's first pixel.
 void launch_tile_row_scan(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* row_prefix, int ntiles,
                           std::uint64_t total_rows, std::uint64_t* pix, std::uint64_t* row_cells,
-                          std::uint32_t* row_max, hipStream_t s);
-// Exclusive scan of row_cells per tile into row_start; tile_cells[t] = total,
-// tile_max[t] = largest label count of the tile.
+                          std::uint32_t* row_max, std::uint64_t* bstart, hipStream_t s);
+// Band corrections of every tile (see TileDesc): bcorr[band_base + b * h + y];
+// tile_cells[t] = cells of the tile, tile_max[t] = its largest label count.
 void launch_tile_cells(const TileDesc* tiles, int ntiles, const std::uint64_t* row_cells, const std::uint32_t* row_max,
-                       std::uint64_t* row_start, std::uint64_t* tile_cells, std::uint32_t* tile_max, hipStream_t s);
+                       const std::uint64_t* bstart, std::uint64_t* bcorr, std::uint64_t* tile_cells,
+                       std::uint32_t* tile_max, hipStream_t s);
 // Path kernel geometry: a wave64 scans kScanPerWave scanlines with
 // kLanesPerScan lanes each (group prefixes count waves).
 constexpr int kLanesPerScan = 8;
 constexpr int kScanPerWave = 64 / kLanesPerScan;
 // Matching costs (u8) of every tile cell into C (same cell indexing as S).
 void launch_cost(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* row_prefix, int ntiles,
-                 std::uint64_t total_rows, const std::uint64_t* row_start, const std::uint64_t* pix, std::uint8_t* C,
+                 std::uint64_t total_rows, const std::uint64_t* bcorr, const std::uint64_t* pix, std::uint8_t* C,
                  hipStream_t s);
 // One SGM path over every tile (path index 0..7, see kernels_sgm.hip): a
 // persistent launch of at most `slots` waves (path_slots(): the waves that fit
@@ -78,7 +80,7 @@ std::uint64_t path_slots();
 std::size_t path_scratch_elems(std::uint64_t slots, int max_num);
 int path_scratch_row(int max_num);
 void launch_sgm_path(int path, const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* group_prefix, int ntiles,
-                     std::uint64_t groups, const std::uint64_t* row_start, const std::uint64_t* pix,
+                     std::uint64_t groups, const std::uint64_t* bcorr, const std::uint64_t* pix,
                      const std::uint8_t* C, std::uint16_t* S, std::uint16_t* scratch, std::uint64_t slots, int max_num,
                      std::uint32_t* queue, bool accumulate, hipStream_t s);
 // Extra elements S and C hold past their last cell.

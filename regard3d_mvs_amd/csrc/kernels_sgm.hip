@@ -72,7 +72,8 @@ __global__ __launch_bounds__(256) void k_tile_row_scan(const JobDesc* __restrict
                                                        const std::uint64_t* __restrict__ row_prefix, int ntiles,
                                                        std::uint64_t* __restrict__ pix,
                                                        std::uint64_t* __restrict__ row_cells,
-                                                       std::uint32_t* __restrict__ row_max) {
+                                                       std::uint32_t* __restrict__ row_max,
+                                                       std::uint64_t* __restrict__ bstart) {
   const std::uint64_t r = blockIdx.x;
   const int t = find_tile(row_prefix, ntiles, r);
   const TileDesc T = tiles[t];
@@ -101,7 +102,13 @@ __global__ __launch_bounds__(256) void k_tile_row_scan(const JobDesc* __restrict
       sc[threadIdx.x] += add;
       __syncthreads();
     }
-    if (x < T.w) pix[T.px_base + std::uint64_t(y) * T.w + x] = pix_record(carry + sc[threadIdx.x] - v, lo, num);
+    if (x < T.w) {
+      const std::uint32_t off = carry + sc[threadIdx.x] - v;
+      pix[T.px_base + std::uint64_t(y) * T.w + x] = pix_record(off, lo, num);
+      // Row offset of the first pixel of each band.
+      if ((x & ((1 << min(T.band_shift, 30)) - 1)) == 0 && (x >> T.band_shift) < T.nbands)
+        bstart[T.band_base + std::uint64_t(x >> T.band_shift) * T.h + y] = off;
+    }
     carry += sc[255];
     __syncthreads();
   }
@@ -111,11 +118,14 @@ __global__ __launch_bounds__(256) void k_tile_row_scan(const JobDesc* __restrict
   }
 }
 
-// Exclusive scan of the row cell counts of one tile: row_start, tile_cells.
+// Totals of one tile (tile_cells, tile_max) and the band corrections: an
+// exclusive scan of the band-row cell counts in band-major order, minus the
+// row offset of the band's first pixel.
 __global__ __launch_bounds__(256) void k_tile_cells(const TileDesc* __restrict__ tiles,
                                                     const std::uint64_t* __restrict__ row_cells,
                                                     const std::uint32_t* __restrict__ row_max,
-                                                    std::uint64_t* __restrict__ row_start,
+                                                    const std::uint64_t* __restrict__ bstart,
+                                                    std::uint64_t* __restrict__ bcorr,
                                                     std::uint64_t* __restrict__ tile_cells,
                                                     std::uint32_t* __restrict__ tile_max) {
   const TileDesc T = tiles[blockIdx.x];
@@ -123,11 +133,18 @@ __global__ __launch_bounds__(256) void k_tile_cells(const TileDesc* __restrict__
   __shared__ std::uint32_t smax;
   if (threadIdx.x == 0) smax = 0;
   __syncthreads();
+  for (int y = threadIdx.x; y < T.h; y += 256) atomicMax(&smax, row_max[T.row_base + y]);
+  const int n = T.nbands * T.h;
   std::uint64_t carry = 0;
-  for (int base = 0; base < T.h; base += 256) {
-    const int y = base + threadIdx.x;
-    const std::uint64_t v = y < T.h ? row_cells[T.row_base + y] : 0;
-    if (y < T.h) atomicMax(&smax, row_max[T.row_base + y]);
+  for (int base = 0; base < n; base += 256) {
+    const int e = base + threadIdx.x;
+    std::uint64_t v = 0, b0 = 0;
+    if (e < n) {
+      const int b = e / T.h, y = e - b * T.h;
+      b0 = bstart[T.band_base + e];
+      const std::uint64_t b1 = b + 1 < T.nbands ? bstart[T.band_base + e + T.h] : row_cells[T.row_base + y];
+      v = b1 - b0;
+    }
     sc[threadIdx.x] = v;
     __syncthreads();
     for (int k = 1; k < 256; k <<= 1) {
@@ -136,7 +153,7 @@ __global__ __launch_bounds__(256) void k_tile_cells(const TileDesc* __restrict__
       sc[threadIdx.x] += add;
       __syncthreads();
     }
-    if (y < T.h) row_start[T.row_base + y] = carry + sc[threadIdx.x] - v;
+    if (e < n) bcorr[T.band_base + e] = carry + sc[threadIdx.x] - v - b0;
     carry += sc[255];
     __syncthreads();
   }
@@ -189,7 +206,7 @@ __device__ __forceinline__ void scan_start(int s, int w, int h, int& x, int& y, 
 // its pixel by binary search over the row's cell offsets and writes 16 bytes.
 __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, const TileDesc* __restrict__ tiles,
                                               const std::uint64_t* __restrict__ row_prefix, int ntiles,
-                                              const std::uint64_t* __restrict__ row_start,
+                                              const std::uint64_t* __restrict__ bcorr,
                                               const std::uint64_t* __restrict__ pix, std::uint8_t* __restrict__ C) {
   constexpr int kStage = 4096;  // row pixels whose cell offsets are staged in LDS
   __shared__ std::uint32_t soff[kStage];
@@ -214,7 +231,8 @@ __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, 
   const auto cr = gptr(J.cr) + std::size_t(Y) * J.cr_stride;
   const int pw1 = J.pwidth - 1;
   const std::uint32_t invalid = static_cast<std::uint32_t>(J.invalid_cost);
-  std::uint8_t* out = C + T.cell_base + row_start[T.row_base + y];
+  std::uint8_t* out = C + T.cell_base;
+  const std::uint64_t* bc = bcorr + T.band_base + y;
   for (std::uint32_t k = threadIdx.x; k < chunks; k += blockDim.x) {
     // Every pixel holds at least one chunk, so the pixel of chunk k is at most k.
     int xa = 0, xb = min(static_cast<int>(k), T.w - 1);
@@ -246,7 +264,8 @@ __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, 
       }
       w[q] = v;
     }
-    *reinterpret_cast<uint4*>(out + 16 * std::size_t(k)) = make_uint4(w[0], w[1], w[2], w[3]);
+    *reinterpret_cast<uint4*>(out + bc[std::size_t(xa >> T.band_shift) * T.h] + 16 * std::size_t(k)) =
+        make_uint4(w[0], w[1], w[2], w[3]);
   }
 }
 
@@ -310,7 +329,7 @@ struct SegIn {
 // Per-step values of one scanline pixel, loaded two steps ahead.
 struct StepIn {
   std::uint64_t rec;  // pixel record
-  std::uint32_t rs;   // cell offset of the pixel's tile row
+  std::uint32_t rs;   // cell offset of the pixel's band row (band correction)
   std::uint32_t ip;   // base intensity
 };
 // Segments 0 and 1 of a pixel, loaded one step ahead.
@@ -334,7 +353,7 @@ struct StepSegs {
 template <int DX, int DY, bool kAcc>
 __global__ __launch_bounds__(64 * kWaves) void k_path(
     const JobDesc* __restrict__ jobs, const TileDesc* __restrict__ tiles, const std::uint64_t* __restrict__ gprefix,
-    int ntiles, std::uint64_t groups, const std::uint64_t* __restrict__ row_start,
+    int ntiles, std::uint64_t groups, const std::uint64_t* __restrict__ bcorr,
     const std::uint64_t* __restrict__ pix, const std::uint8_t* __restrict__ C, std::uint16_t* __restrict__ S,
     std::uint16_t* __restrict__ scratch, int scratch_row, std::uint32_t* __restrict__ queue) {
   __shared__ __attribute__((aligned(16))) std::uint16_t lut[kWaves][256];
@@ -394,7 +413,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
 
   const std::uint32_t P1x2 = static_cast<std::uint32_t>(J.p1) * 0x10001U;
   const std::uint64_t* tpix = pix + T.px_base;
-  const std::uint64_t* trs = row_start + T.row_base;
+  const std::uint64_t* tbc = bcorr + T.band_base;
   const std::uint16_t* img = J.img;
   const int img_stride = static_cast<int>(J.img_stride);
   const std::uint8_t* Ct = C + T.cell_base;
@@ -404,7 +423,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
     i = min(i, len - 1);
     const int x = x0 + i * DX, y = y0 + i * DY;
     u.rec = ldb<std::uint64_t>(tpix, static_cast<std::uint32_t>(y * T.w + x) * 8U);
-    u.rs = ldb<std::uint32_t>(trs, static_cast<std::uint32_t>(y) * 8U);
+    u.rs = ldb<std::uint32_t>(tbc, static_cast<std::uint32_t>((x >> T.band_shift) * T.h + y) * 8U);
     u.ip = ldb<std::uint16_t>(img, static_cast<std::uint32_t>((T.y0 + y) * img_stride + T.x0 + x) * 2U);
   };
   // Costs and S of segment s (lanes past the pixel's labels read its last ones;
@@ -700,7 +719,7 @@ __device__ __forceinline__ void wta(const std::uint16_t* __restrict__ s, int n, 
 // select_disparities (stereo_matcher.cpp:446-525) for one tile row per block.
 __global__ __launch_bounds__(256) void k_select(const JobDesc* __restrict__ jobs, const TileDesc* __restrict__ tiles,
                                                 const std::uint64_t* __restrict__ row_prefix, int ntiles,
-                                                const std::uint64_t* __restrict__ row_start,
+                                                const std::uint64_t* __restrict__ bcorr,
                                                 const std::uint64_t* __restrict__ pix,
                                                 const std::uint16_t* __restrict__ S, SelectParams p,
                                                 float* __restrict__ tdisp, float* __restrict__ tconf,
@@ -716,7 +735,8 @@ __global__ __launch_bounds__(256) void k_select(const JobDesc* __restrict__ jobs
   const int lf = rows.x, ll = rows.y, rf = rows.z, rl = rows.w;
   const int xb = min(max(lf - T.x0, 0), T.w);
   const int xe = min(max(ll + 1 - T.x0, xb), T.w);
-  const std::uint64_t rowcell = T.cell_base + row_start[T.row_base + y];
+  const std::uint64_t* bc = bcorr + T.band_base + y;
+  auto cells_of = [&](int x) { return S + T.cell_base + bc[std::size_t(x >> T.band_shift) * T.h]; };
   const std::uint64_t px = T.px_base + std::uint64_t(y) * T.w;
   const int D = J.D;
 
@@ -728,7 +748,7 @@ __global__ __launch_bounds__(256) void k_select(const JobDesc* __restrict__ jobs
     for (int x = xb + threadIdx.x; x < xe; x += blockDim.x) {
       const std::uint64_t rec = pix[px + x];
       const int lo = pix_lo(rec), num = pix_num(rec);
-      const std::uint16_t* s = S + rowcell + pix_off(rec);
+      const std::uint16_t* s = cells_of(x) + pix_off(rec);
       for (int k = 0; k < num; ++k) {
         atomicMin(&keys[x - (lo + k) + D - 1], (std::uint32_t(s[k]) << 16) | std::uint32_t(x));
       }
@@ -743,7 +763,7 @@ __global__ __launch_bounds__(256) void k_select(const JobDesc* __restrict__ jobs
       const int X = T.x0 + x;
       const std::uint64_t rec = pix[px + x];
       const int lo = pix_lo(rec), n = pix_num(rec);
-      const std::uint16_t* s = S + rowcell + pix_off(rec);
+      const std::uint16_t* s = cells_of(x) + pix_off(rec);
       int kb;
       std::uint32_t bc, second;
       wta(s, n, kb, bc, second);
@@ -903,21 +923,22 @@ void path_launch(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t
 
 void launch_tile_row_scan(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* row_prefix, int ntiles,
                           std::uint64_t total_rows, std::uint64_t* pix, std::uint64_t* row_cells,
-                          std::uint32_t* row_max, hipStream_t s) {
+                          std::uint32_t* row_max, std::uint64_t* bstart, hipStream_t s) {
   k_tile_row_scan<<<dim3(static_cast<unsigned>(total_rows)), 256, 0, s>>>(jobs, tiles, row_prefix, ntiles, pix,
-                                                                          row_cells, row_max);
+                                                                          row_cells, row_max, bstart);
 }
 
 void launch_tile_cells(const TileDesc* tiles, int ntiles, const std::uint64_t* row_cells, const std::uint32_t* row_max,
-                       std::uint64_t* row_start, std::uint64_t* tile_cells, std::uint32_t* tile_max, hipStream_t s) {
-  k_tile_cells<<<dim3(ntiles), 256, 0, s>>>(tiles, row_cells, row_max, row_start, tile_cells, tile_max);
+                       const std::uint64_t* bstart, std::uint64_t* bcorr, std::uint64_t* tile_cells,
+                       std::uint32_t* tile_max, hipStream_t s) {
+  k_tile_cells<<<dim3(ntiles), 256, 0, s>>>(tiles, row_cells, row_max, bstart, bcorr, tile_cells, tile_max);
 }
 
 void launch_cost(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* row_prefix, int ntiles,
-                 std::uint64_t total_rows, const std::uint64_t* row_start, const std::uint64_t* pix, std::uint8_t* C,
+                 std::uint64_t total_rows, const std::uint64_t* bcorr, const std::uint64_t* pix, std::uint8_t* C,
                  hipStream_t s) {
   if (total_rows == 0) return;
-  k_cost<<<dim3(static_cast<unsigned>(total_rows)), 256, 0, s>>>(jobs, tiles, row_prefix, ntiles, row_start, pix, C);
+  k_cost<<<dim3(static_cast<unsigned>(total_rows)), 256, 0, s>>>(jobs, tiles, row_prefix, ntiles, bcorr, pix, C);
 }
 
 int path_scratch_row(int max_num) { return kPad + (max_num + 15) / 16 * 16 + kPad; }

@@ -394,7 +394,7 @@ struct Aggregation {
   std::vector<Grid> grids;
   std::vector<int> tile_base;
   std::vector<std::uint64_t> tcells;
-  DBuf<std::uint64_t> drow_prefix, row_start, pix;
+  DBuf<std::uint64_t> drow_prefix, bcorr, pix;  // bcorr: band corrections (TileDesc)
   std::uint64_t total_rows = 0, total_px = 0, total_cells = 0, max_tile_cells = 0;
   int max_w = 1, max_D = 16, max_num = 16;
 };
@@ -409,7 +409,9 @@ struct Chunk {
   DBuf<std::uint16_t> S;
 };
 
-Aggregation layout_jobs(Context& c, std::vector<Job>& jobs, const r3dg_options& o) {
+// band_shift: log2 of the volume band width (TileDesc); 31 stores every tile
+// row contiguously (the packed order of detail::sgm_cost_volume).
+Aggregation layout_jobs(Context& c, std::vector<Job>& jobs, const r3dg_options& o, int band_shift = 3) {
   Aggregation A;
   const int nj = static_cast<int>(jobs.size());
   const auto lut = p2_table(o);
@@ -453,7 +455,8 @@ Aggregation layout_jobs(Context& c, std::vector<Job>& jobs, const r3dg_options& 
   std::vector<std::uint32_t> tmax;
   DBuf<TileDesc>& dtiles = A.dtiles;
   DBuf<std::uint64_t>& drow_prefix = A.drow_prefix;
-  DBuf<std::uint64_t>& row_start = A.row_start;
+  DBuf<std::uint64_t>& bcorr = A.bcorr;
+  DBuf<std::uint64_t> bstart;
   DBuf<std::uint64_t>& pix = A.pix;
   DBuf<std::uint64_t> row_cells, tile_cells;
   DBuf<std::uint32_t> row_max, tile_max;
@@ -466,6 +469,7 @@ Aggregation layout_jobs(Context& c, std::vector<Job>& jobs, const r3dg_options& 
     row_prefix.clear();
     total_rows = 0;
     total_px = 0;
+    std::uint64_t total_bands = 0;
     for (int j = 0; j < nj; ++j) {
       const Job& J = jobs[std::size_t(j)];
       grids[std::size_t(j)] = layout(J.base->w, J.base->h, tsize[std::size_t(j)], ext);
@@ -480,9 +484,13 @@ Aggregation layout_jobs(Context& c, std::vector<Job>& jobs, const r3dg_options& 
           t.job = j;
           t.px_base = total_px;
           t.row_base = total_rows;
+          t.band_shift = band_shift;
+          t.nbands = band_shift >= 30 ? 1 : (t.w + (1 << band_shift) - 1) >> band_shift;
+          t.band_base = total_bands;
           row_prefix.push_back(total_rows);
           total_px += std::uint64_t(t.w) * std::uint64_t(t.h);
           total_rows += std::uint64_t(t.h);
+          total_bands += std::uint64_t(t.nbands) * std::uint64_t(t.h);
           td.push_back(t);
         }
       }
@@ -494,13 +502,14 @@ Aggregation layout_jobs(Context& c, std::vector<Job>& jobs, const r3dg_options& 
     pix = DBuf<std::uint64_t>(c.arena, total_px);
     row_cells = DBuf<std::uint64_t>(c.arena, total_rows);
     row_max = DBuf<std::uint32_t>(c.arena, total_rows);
-    row_start = DBuf<std::uint64_t>(c.arena, total_rows);
+    bstart = DBuf<std::uint64_t>(c.arena, total_bands);
+    bcorr = DBuf<std::uint64_t>(c.arena, total_bands);
     tile_cells = DBuf<std::uint64_t>(c.arena, std::size_t(nt));
     tile_max = DBuf<std::uint32_t>(c.arena, std::size_t(nt));
     launch_tile_row_scan(djobs.get(), dtiles.get(), drow_prefix.get(), nt, total_rows, pix.get(), row_cells.get(),
-                         row_max.get(), c.stream);
+                         row_max.get(), bstart.get(), c.stream);
     check_launch();
-    launch_tile_cells(dtiles.get(), nt, row_cells.get(), row_max.get(), row_start.get(), tile_cells.get(),
+    launch_tile_cells(dtiles.get(), nt, row_cells.get(), row_max.get(), bstart.get(), bcorr.get(), tile_cells.get(),
                       tile_max.get(), c.stream);
     check_launch();
     tcells.assign(std::size_t(nt), 0);
@@ -596,7 +605,7 @@ Chunk aggregate_tiles(Context& c, Aggregation& A, int t0, int t1) {
   DBuf<std::uint8_t> C(c.arena, ch.cells + kSPad);
   {
     const StageTimer timer(c, R3DG_STAGE_PREPARE);
-    launch_cost(A.djobs.get(), ch.dtiles.get(), ch.drow_prefix.get(), nt, ch.rows, A.row_start.get(), A.pix.get(),
+    launch_cost(A.djobs.get(), ch.dtiles.get(), ch.drow_prefix.get(), nt, ch.rows, A.bcorr.get(), A.pix.get(),
                 C.get(), c.stream);
     check_launch();
   }
@@ -606,7 +615,7 @@ Chunk aggregate_tiles(Context& c, Aggregation& A, int t0, int t1) {
     for (int p = 0; p < 8; ++p) {
       const std::size_t o8 = std::size_t(p) * (std::size_t(nt) + 1);
       launch_sgm_path(p, A.djobs.get(), ch.dtiles.get(), dgp.get() + o8, nt, gp[o8 + std::size_t(nt)],
-                      A.row_start.get(), A.pix.get(), C.get(), ch.S.get(), scratch.get(), slots, max_num,
+                      A.bcorr.get(), A.pix.get(), C.get(), ch.S.get(), scratch.get(), slots, max_num,
                       queues.get() + p, p > 0, c.stream);
       check_launch();
     }
@@ -656,7 +665,7 @@ std::vector<TilesOut> match_tiles(Context& c, std::vector<Job>& jobs, const r3dg
     Chunk ch = aggregate_tiles(c, A, t0, t1);
     // Selection into per-tile maps.
     const StageTimer timer(c, R3DG_STAGE_SELECT);
-    launch_select(A.djobs.get(), ch.dtiles.get(), ch.drow_prefix.get(), t1 - t0, ch.rows, A.row_start.get(),
+    launch_select(A.djobs.get(), ch.dtiles.get(), ch.drow_prefix.get(), t1 - t0, ch.rows, A.bcorr.get(),
                   A.pix.get(), ch.S.get(), sp, A.max_w, A.max_D, tdisp.get(), tconf.get(), tlab.get(), c.stream);
     check_launch();
   }
@@ -1478,7 +1487,7 @@ int32_t r3dg_cost_volume(r3dg_ctx* h, r3dg_image left, r3dg_image right, int32_t
     jobs[0].ranges = has ? &rg : nullptr;
     jobs[0].dmin = disparity_min;
     jobs[0].D = D;
-    Aggregation A = layout_jobs(c, jobs, one);
+    Aggregation A = layout_jobs(c, jobs, one, 31);  // rows contiguous: the reference's packed order
     if (A.total_cells != out_cells) fail(R3DG_ERROR_INTERNAL, "cost_volume: cell count mismatch");
     const Chunk ch = aggregate_tiles(c, A, 0, static_cast<int>(A.td.size()));
     const hipMemcpyKind kind = out_memory == R3DG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;

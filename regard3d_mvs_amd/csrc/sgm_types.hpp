@@ -58,13 +58,20 @@ struct JobDesc {
   std::uint16_t lut[256];    // P2 by 8-bit intensity step (stereo_matcher.cpp:69-79)
 };
 
+// Cells of a tile are stored band by band: columns [b << band_shift,
+// (b + 1) << band_shift) of every row, rows in order, then the next band.
+// Pixel (x, y)'s labels start at cell_base + bcorr[band_base + (x >> band_shift)
+// * h + y] + off(x, y), off being the pixel record's offset in its row.
 struct TileDesc {
   std::int32_t x0, y0, w, h;  // in the job's base image
   std::int32_t job;
   std::int32_t max_num;       // largest label count of the tile
   std::uint64_t cell_base;    // first cell of the tile in S
   std::uint64_t px_base;      // first tile pixel in per-pixel arrays
-  std::uint64_t row_base;     // first tile row in row_start
+  std::uint64_t row_base;     // first tile row in per-row arrays
+  std::int32_t band_shift;    // log2 of the band width (31: one band per row)
+  std::int32_t nbands;
+  std::uint64_t band_base;    // first (band, row) entry in bcorr
 };
 
 // Tile grid of one job for blending (stereo_matcher.cpp:688-766): tile (i, j)
