@@ -276,12 +276,30 @@ __global__ __launch_bounds__(256) void k_cc_border(const float* __restrict__ d, 
   if (ty == kCcH - 1 && y + 1 < h && fabsf(d[i + w] - d[i]) <= max_diff) cc_union(p, i, i + w);
 }
 
-__global__ void k_cc_count(int* p, int* __restrict__ size, int n, int min_size) {
+// Component sizes, counted up to min_size: the lanes of a wave that share a
+// root add their count with one atomic.  With a finite max_diff, non-finite
+// pixels are components of their own (|inf - v| > max_diff) and stay at 0,
+// below any min_size > 1, as their count of 1 would; with an infinite one
+// they can join a segment and are counted.
+__global__ __launch_bounds__(256) void k_cc_count(const float* __restrict__ d, int* p, int* __restrict__ size, int n,
+                                                  int min_size, bool count_nonfinite) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int r = cc_find(p, i);
-  __hip_atomic_store(&p[i], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (ld_relaxed(&size[r]) < min_size) atomicAdd(&size[r], 1);
+  const int lane = threadIdx.x & 63;
+  int r = -1;
+  bool live = false;
+  if (i < n) {
+    r = cc_find(p, i);
+    __hip_atomic_store(&p[i], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    live = count_nonfinite || finite(d[i]);
+  }
+  std::uint64_t todo = __builtin_amdgcn_ballot_w64(live);
+  while (todo != 0) {
+    const int leader = __builtin_ctzll(todo);
+    const int lr = __shfl(r, leader);
+    const std::uint64_t same = __builtin_amdgcn_ballot_w64(live && r == lr) & todo;
+    if (lane == leader && ld_relaxed(&size[lr]) < min_size) atomicAdd(&size[lr], __builtin_popcountll(same));
+    todo &= ~same;
+  }
 }
 
 __global__ void k_cc_apply(float* __restrict__ d, const int* __restrict__ p, const int* __restrict__ size, int n,
@@ -533,7 +551,7 @@ void launch_speckles(float* d, int w, int h, int min_size, float max_diff, std::
   const dim3 b(kCcW, kCcH), g((w + kCcW - 1) / kCcW, (h + kCcH - 1) / kCcH);
   k_cc_local<<<g, b, 0, s>>>(d, w, h, max_diff, parent, size);
   k_cc_border<<<g, b, 0, s>>>(d, w, h, max_diff, parent);
-  k_cc_count<<<(n + 255) / 256, 256, 0, s>>>(parent, size, n, min_size);
+  k_cc_count<<<(n + 255) / 256, 256, 0, s>>>(d, parent, size, n, min_size, !(max_diff <= 3.4e38F));
   k_cc_apply<<<(n + 255) / 256, 256, 0, s>>>(d, parent, size, n, min_size);
 }
 

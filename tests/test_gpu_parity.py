@@ -146,7 +146,23 @@ def test_speckles_and_median(gpu, seed):
     d[rng.random(d.shape) < 0.15] = np.inf
     for size in (2, 7, 40):
         same_bits(gpu.remove_speckles(d, size, 1.0), O.remove_speckles(d, size, 1.0))
+    # An infinite step joins invalid (+inf) pixels to their finite neighbours'
+    # segments, as |inf - v| <= inf in the reference's segment growth.
+    for size in (3, 200, 3000):
+        same_bits(gpu.remove_speckles(d, size, np.inf), O.remove_speckles(d, size, np.inf))
     same_bits(gpu.median_filter_3x3(d), O.median_filter_3x3(d))
+
+
+def test_speckles_large_segments(gpu):
+    """Segments spanning many blocks (wave-aggregated counting near the
+    saturation threshold)."""
+    rng = np.random.default_rng(7)
+    h, w = 300, 520
+    d = np.full((h, w), 10.0, np.float32)
+    d[:, 200:] += np.cumsum(rng.random((h, w - 200)).astype(np.float32) * 0.2, axis=1)
+    d[rng.random(d.shape) < 0.05] = np.inf
+    for size in (50, 4000, 60000, 150000):
+        same_bits(gpu.remove_speckles(d, size, 0.3), O.remove_speckles(d, size, 0.3))
 
 
 # ---- complete matcher
