@@ -209,7 +209,9 @@ __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, 
                                               const std::uint64_t* __restrict__ bcorr,
                                               const std::uint64_t* __restrict__ pix, std::uint8_t* __restrict__ C) {
   constexpr int kStage = 4096;  // row pixels whose cell offsets are staged in LDS
+  constexpr int kWin = 2048;    // partner Census columns staged in LDS
   __shared__ std::uint32_t soff[kStage];
+  __shared__ std::uint64_t swin[kWin];
   const std::uint64_t r = blockIdx.x;
   const int t = find_tile(row_prefix, ntiles, r);
   const TileDesc T = tiles[t];
@@ -232,6 +234,17 @@ __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, 
   const int pw1 = J.pwidth - 1;
   const std::uint32_t invalid = static_cast<std::uint32_t>(J.invalid_cost);
   std::uint8_t* out = C + T.cell_base;
+  // Partner columns a cost of this row can use: X - dmin - label for X in the
+  // tile and label in [0, D); every valid column lies in [c0, c1].
+  const int c0 = max(0, T.x0 - J.dmin - (J.D - 1)), c1 = min(pw1, T.x0 + T.w - 1 - J.dmin);
+  const bool windowed = c1 >= c0 && c1 - c0 < kWin;
+  if (windowed) {
+    for (int k = threadIdx.x; k <= c1 - c0; k += blockDim.x) swin[k] = cr[c0 + k];
+  }
+  __syncthreads();
+  auto partner = [&](int xr) {
+    return windowed ? swin[min(max(xr, c0), c1) - c0] : cr[min(max(xr, 0), pw1)];
+  };
   const std::uint64_t* bc = bcorr + T.band_base + y;
   for (std::uint32_t k = threadIdx.x; k < chunks; k += blockDim.x) {
     // Every pixel holds at least one chunk, so the pixel of chunk k is at most k.
@@ -257,7 +270,7 @@ __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, 
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         const int xr = xr0 - 4 * q - b;
-        std::uint32_t cost = xr >= pfirst && xr <= plast ? static_cast<std::uint32_t>(__popcll(c ^ cr[min(max(xr, 0), pw1)]))
+        std::uint32_t cost = xr >= pfirst && xr <= plast ? static_cast<std::uint32_t>(__popcll(c ^ partner(xr)))
                                                          : invalid;
         cost = lvalid ? cost : 0U;
         v |= cost << (8 * b);
