@@ -42,6 +42,24 @@ __device__ __forceinline__ int find_tile(const std::uint64_t* __restrict__ prefi
   return lo;
 }
 
+// The same search by a whole wave (every lane active, the same id in every
+// lane): each round samples 64 evenly spaced prefixes and keeps the interval
+// between the last sample <= id and the next, so 4096 tiles take two rounds
+// of loads instead of twelve.
+__device__ __forceinline__ int find_tile_wave(const std::uint64_t* __restrict__ prefix, int ntiles, std::uint64_t id) {
+  const int lane = static_cast<int>(__lane_id());
+  int lo = 0, span = ntiles;  // prefix[lo] <= id; the tile lies in [lo, lo + span)
+  while (span > 1) {
+    const int step = (span + 63) >> 6;
+    const int i = lane * step;
+    const bool le = i < span && prefix[lo + i] <= id;
+    const int c = __popcll(__ballot(le));
+    lo += (c - 1) * step;
+    span = min(step, span - (c - 1) * step);
+  }
+  return lo;
+}
+
 // Pointers read out of JobDesc / TileDesc are generic to the compiler; these
 // views make their loads and stores global_* instructions (flat ones also count
 // against the LDS wait counter, which would serialise the prefetch below).
@@ -75,7 +93,7 @@ __global__ __launch_bounds__(256) void k_tile_row_scan(const JobDesc* __restrict
                                                        std::uint32_t* __restrict__ row_max,
                                                        std::uint64_t* __restrict__ bstart) {
   const std::uint64_t r = blockIdx.x;
-  const int t = find_tile(row_prefix, ntiles, r);
+  const int t = find_tile_wave(row_prefix, ntiles, r);
   const TileDesc T = tiles[t];
   const JobDesc& J = jobs[T.job];
   const int y = static_cast<int>(r - row_prefix[t]);
@@ -213,7 +231,7 @@ __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, 
   __shared__ std::uint32_t soff[kStage];
   __shared__ std::uint64_t swin[kWin];
   const std::uint64_t r = blockIdx.x;
-  const int t = find_tile(row_prefix, ntiles, r);
+  const int t = find_tile_wave(row_prefix, ntiles, r);
   const TileDesc T = tiles[t];
   const JobDesc& J = jobs[T.job];
   const int y = static_cast<int>(r - row_prefix[t]);
@@ -698,38 +716,51 @@ __device__ __forceinline__ float subpixel_offset(int method, float sm, float s0,
   return delta < -0.5F ? -0.5F : (0.5F < delta ? 0.5F : delta);
 }
 
-// First argmin k_best, its cost, and the lowest cost outside k_best +- 1
-// (sgm_kernels.cpp:405-426).
-__device__ __forceinline__ void wta(const std::uint16_t* __restrict__ s, int n, int& kb, std::uint32_t& bc,
-                                    std::uint32_t& sec) {
-  kb = 0;
-  bc = 0xFFFFFFFFU;
-  for (int k0 = 0; k0 < n; k0 += 8) {
-    const uint4 q = *reinterpret_cast<const uint4*>(s + k0);
-    const std::uint32_t w[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const std::uint32_t v = (w[e >> 1] >> ((e & 1) * 16)) & 0xFFFFU;
-      if (v < bc) {
-        bc = v;
-        kb = k0 + e;
-      }
-    }
-  }
-  sec = 0xFFFFU;
-  for (int k0 = 0; k0 < n; k0 += 8) {
-    const uint4 q = *reinterpret_cast<const uint4*>(s + k0);
-    const std::uint32_t w[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int k = k0 + e;
-      const std::uint32_t v = (w[e >> 1] >> ((e & 1) * 16)) & 0xFFFFU;
-      if (k + 1 < kb || k > kb + 1) sec = min(sec, v);
-    }
-  }
+// select_disparities (stereo_matcher.cpp:446-525) for one tile row per block,
+// four lanes (a quad) per pixel.  Lane q of a quad holds labels
+// [32 i + 8 q, 32 i + 8 q + 8) of chunk i, so a quad reads 64 contiguous
+// bytes per chunk; the first two chunks stay in registers for the second
+// pass of the winner-takes-all (sgm_kernels.cpp:405-426).
+__device__ __forceinline__ std::uint32_t quad_min(std::uint32_t v) {
+  const int i = static_cast<int>(v);
+  v = min(v, static_cast<std::uint32_t>(__builtin_amdgcn_update_dpp(i, i, 0xB1, 0xF, 0xF, false)));
+  const int j = static_cast<int>(v);
+  return min(v, static_cast<std::uint32_t>(__builtin_amdgcn_update_dpp(j, j, 0x4E, 0xF, 0xF, false)));
 }
 
-// select_disparities (stereo_matcher.cpp:446-525) for one tile row per block.
+// Minimum of (value << 16 | label) over the 8 labels of a chunk word group,
+// labels k0..k0+7: the lowest value, the first label on ties.
+__device__ __forceinline__ std::uint32_t key_min(const uint4& q, int k0, std::uint32_t m) {
+  const std::uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const std::uint32_t v = (w[e >> 1] >> ((e & 1) * 16)) & 0xFFFFU;
+    m = min(m, (v << 16) | static_cast<std::uint32_t>(k0 + e));
+  }
+  return m;
+}
+
+// Lowest value of labels k0..k0+7 outside [kb - 1, kb + 1].
+__device__ __forceinline__ std::uint32_t outside_min(const uint4& q, int k0, int kb, std::uint32_t m) {
+  const std::uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const std::uint32_t v = (w[e >> 1] >> ((e & 1) * 16)) & 0xFFFFU;
+    const int k = k0 + e;
+    m = k + 1 < kb || k > kb + 1 ? min(m, v) : m;
+  }
+  return m;
+}
+
+// Label k of a lane's group of 8 starting at k0 (0xFFFFFFFF when the group
+// does not hold k).
+__device__ __forceinline__ std::uint32_t group_value(const uint4& q, int k0, int k) {
+  const int e = k - k0;
+  const std::uint64_t a = (std::uint64_t(q.y) << 32) | q.x, b = (std::uint64_t(q.w) << 32) | q.z;
+  const std::uint32_t v = static_cast<std::uint32_t>((e < 4 ? a : b) >> ((e & 3) * 16)) & 0xFFFFU;
+  return e >= 0 && e < 8 ? v : 0xFFFFFFFFU;
+}
+
 __global__ __launch_bounds__(256) void k_select(const JobDesc* __restrict__ jobs, const TileDesc* __restrict__ tiles,
                                                 const std::uint64_t* __restrict__ row_prefix, int ntiles,
                                                 const std::uint64_t* __restrict__ bcorr,
@@ -737,49 +768,101 @@ __global__ __launch_bounds__(256) void k_select(const JobDesc* __restrict__ jobs
                                                 const std::uint16_t* __restrict__ S, SelectParams p,
                                                 float* __restrict__ tdisp, float* __restrict__ tconf,
                                                 std::int16_t* __restrict__ tlab) {
+  constexpr int kStage = 1024;  // row pixels whose first cell and range are staged in LDS
+  __shared__ std::uint64_t scell[kStage];
+  __shared__ std::uint32_t srange[kStage];
   extern __shared__ std::uint32_t keys[];
   const std::uint64_t r = blockIdx.x;
-  const int t = find_tile(row_prefix, ntiles, r);
+  const int t = find_tile_wave(row_prefix, ntiles, r);
   const TileDesc T = tiles[t];
   const JobDesc& J = jobs[T.job];
   const int y = static_cast<int>(r - row_prefix[t]);
   const int Y = T.y0 + y;
-  const int4 rows = reinterpret_cast<const int4*>(J.rows4)[Y];
-  const int lf = rows.x, ll = rows.y, rf = rows.z, rl = rows.w;
+  const auto rows = gptr(J.rows4) + 4 * Y;
+  const int lf = rows[0], ll = rows[1], rf = rows[2], rl = rows[3];
   const int xb = min(max(lf - T.x0, 0), T.w);
   const int xe = min(max(ll + 1 - T.x0, xb), T.w);
   const std::uint64_t* bc = bcorr + T.band_base + y;
-  auto cells_of = [&](int x) { return S + T.cell_base + bc[std::size_t(x >> T.band_shift) * T.h]; };
+  const std::uint16_t* St = S + T.cell_base;
   const std::uint64_t px = T.px_base + std::uint64_t(y) * T.w;
   const int D = J.D;
+  const int q = threadIdx.x & 3;
+  constexpr int kPerPass = 256 / 4;
+  const bool staged = T.w <= kStage;
+  if (staged) {
+    for (int x = threadIdx.x; x < T.w; x += blockDim.x) {
+      const std::uint64_t rec = pix[px + x];
+      scell[x] = bc[std::size_t(x >> T.band_shift) * T.h] + pix_off(rec);
+      srange[x] = static_cast<std::uint32_t>(rec >> 32);
+    }
+    __syncthreads();
+  }
+  // First cell, lo and label count of pixel x.
+  auto pixel = [&](int x, int& lo, int& n) {
+    std::uint64_t cell;
+    std::uint32_t range;
+    if (staged) {
+      cell = scell[x];
+      range = srange[x];
+    } else {
+      const std::uint64_t rec = pix[px + x];
+      cell = bc[std::size_t(x >> T.band_shift) * T.h] + pix_off(rec);
+      range = static_cast<std::uint32_t>(rec >> 32);
+    }
+    lo = static_cast<int>(range & 0xFFFFU);
+    n = static_cast<int>(range >> 16);
+    return St + cell;
+  };
 
   if (p.diagonal_lr) {
     // Right-image WTA along x - label (sgm_kernels.cpp:427-435): the first
     // (in x order) minimum of each diagonal, as a (cost, x) key minimum.
     for (int i = threadIdx.x; i < T.w + D; i += blockDim.x) keys[i] = 0xFFFFFFFFU;
     __syncthreads();
-    for (int x = xb + threadIdx.x; x < xe; x += blockDim.x) {
-      const std::uint64_t rec = pix[px + x];
-      const int lo = pix_lo(rec), num = pix_num(rec);
-      const std::uint16_t* s = cells_of(x) + pix_off(rec);
-      for (int k = 0; k < num; ++k) {
-        atomicMin(&keys[x - (lo + k) + D - 1], (std::uint32_t(s[k]) << 16) | std::uint32_t(x));
+    for (int x = xb + (threadIdx.x >> 2); x < xe; x += kPerPass) {
+      int lo, num;
+      const std::uint16_t* s = pixel(x, lo, num);
+      for (int k0 = 8 * q; k0 < num; k0 += 32) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          atomicMin(&keys[x - (lo + k0 + e) + D - 1], (std::uint32_t(s[k0 + e]) << 16) | std::uint32_t(x));
       }
     }
     __syncthreads();
   }
 
-  for (int x = threadIdx.x; x < T.w; x += blockDim.x) {
+  for (int base = 0; base < T.w; base += kPerPass) {
+    const int x = base + (threadIdx.x >> 2);
+    if (x >= T.w) continue;  // whole quads leave together
     float d = kInf, conf = 0.0F;
     std::int16_t lab = INT16_MIN;
     if (x >= xb && x < xe) {
       const int X = T.x0 + x;
-      const std::uint64_t rec = pix[px + x];
-      const int lo = pix_lo(rec), n = pix_num(rec);
-      const std::uint16_t* s = cells_of(x) + pix_off(rec);
-      int kb;
-      std::uint32_t bc, second;
-      wta(s, n, kb, bc, second);
+      int lo, n;
+      const std::uint16_t* s = pixel(x, lo, n);
+      // Labels are a multiple of 16 per pixel, so each lane's group of 8 is
+      // either wholly inside the pixel's labels or wholly past them.
+      const int k0 = 8 * q, k1 = 32 + 8 * q;
+      uint4 g0 = make_uint4(0xFFFFFFFFU, 0xFFFFFFFFU, 0xFFFFFFFFU, 0xFFFFFFFFU), g1 = g0;
+      if (k0 < n) g0 = *reinterpret_cast<const uint4*>(s + k0);
+      if (k1 < n) g1 = *reinterpret_cast<const uint4*>(s + k1);
+      std::uint32_t m = 0xFFFFFFFFU;
+      if (k0 < n) m = key_min(g0, k0, m);
+      if (k1 < n) m = key_min(g1, k1, m);
+      for (int k = 64 + 8 * q; k < n; k += 32) m = key_min(*reinterpret_cast<const uint4*>(s + k), k, m);
+      m = quad_min(m);
+      const int kb = static_cast<int>(m & 0xFFFFU);
+      std::uint32_t second = 0xFFFFU;
+      if (k0 < n) second = outside_min(g0, k0, kb, second);
+      if (k1 < n) second = outside_min(g1, k1, kb, second);
+      for (int k = 64 + 8 * q; k < n; k += 32) second = outside_min(*reinterpret_cast<const uint4*>(s + k), k, kb, second);
+      second = quad_min(second);
+      // Costs at kb - 1 and kb + 1 from the registers of the quad when they
+      // lie in the first 64 labels.
+      const std::uint32_t sm = quad_min(min(group_value(g0, k0, kb - 1), group_value(g1, k1, kb - 1)));
+      const std::uint32_t sp = quad_min(min(group_value(g0, k0, kb + 1), group_value(g1, k1, kb + 1)));
+      if (q != 0) continue;
+      const std::uint32_t bcost = m >> 16;
       const int best = lo + kb;
       const int j = kb;
       const int xr = X - J.dmin - best;
@@ -790,18 +873,23 @@ __global__ __launch_bounds__(256) void k_select(const JobDesc* __restrict__ jobs
         const int rbest = static_cast<int>(key & 0xFFFFU) - (x - best);
         if (abs(rbest - best) > p.lr_max_diff) ok = false;
       }
-      const float best_cost = static_cast<float>(bc);
+      const float best_cost = static_cast<float>(bcost);
       if (ok && p.uniqueness > 0.0F && second != 0xFFFFU &&
           best_cost > (1.0F - p.uniqueness) * static_cast<float>(second))
         ok = false;
       if (ok) {
         float delta = 0.0F;
-        if (j > 0 && j < n - 1) delta = subpixel_offset(p.subpixel, static_cast<float>(s[j - 1]), best_cost,
-                                                         static_cast<float>(s[j + 1]));
+        if (j > 0 && j < n - 1) {
+          const std::uint32_t vm = j - 1 < 64 ? sm : s[j - 1];
+          const std::uint32_t vp = j + 1 < 64 ? sp : s[j + 1];
+          delta = subpixel_offset(p.subpixel, static_cast<float>(vm), best_cost, static_cast<float>(vp));
+        }
         d = static_cast<float>(J.dmin + best) + delta;
         conf = second != 0xFFFFU && second > 0 ? 1.0F - best_cost / static_cast<float>(second) : 0.0F;
         lab = static_cast<std::int16_t>(J.dmin + best);
       }
+    } else if (q != 0) {
+      continue;
     }
     tdisp[px + x] = d;
     if (J.confidence) tconf[px + x] = conf;
