@@ -220,15 +220,25 @@ __device__ __forceinline__ void scan_start(int s, int w, int h, int& x, int& y, 
 // the Hamming distance of the base Census and the partner Census at column
 // X - dmin - lo - j when that column lies in the partner row's valid interval,
 // invalid_cost when it does not, 0 for base pixels outside the base row's
-// valid interval.  One block per tile row; a thread per 16-label chunk finds
-// its pixel by binary search over the row's cell offsets and writes 16 bytes.
+// valid interval.  One block per tile row; a thread per 16-label chunk writes
+// 16 bytes.  Rows of at most kPx pixels stage everything a chunk reads in
+// LDS (pixel cell offsets and lo, base Census, band corrections, a
+// chunk-to-pixel map, the partner Census window), so the chunk loop touches
+// global memory only to store; wider rows search their pixel records in
+// global memory.
 __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, const TileDesc* __restrict__ tiles,
                                               const std::uint64_t* __restrict__ row_prefix, int ntiles,
                                               const std::uint64_t* __restrict__ bcorr,
                                               const std::uint64_t* __restrict__ pix, std::uint8_t* __restrict__ C) {
-  constexpr int kStage = 4096;  // row pixels whose cell offsets are staged in LDS
-  constexpr int kWin = 2048;    // partner Census columns staged in LDS
-  __shared__ std::uint32_t soff[kStage];
+  constexpr int kPx = 1024;         // row pixels staged in LDS
+  constexpr int kBands = kPx / 8;   // band corrections staged (8-column bands)
+  constexpr int kMap = 4096;        // chunks mapped to their pixel
+  constexpr int kWin = 2048;        // partner Census columns staged
+  __shared__ std::uint32_t soff[kPx];
+  __shared__ std::uint16_t slo[kPx];
+  __shared__ std::uint64_t scl[kPx];
+  __shared__ std::uint64_t sbc[kBands];
+  __shared__ std::uint16_t smap[kMap];
   __shared__ std::uint64_t swin[kWin];
   const std::uint64_t r = blockIdx.x;
   const int t = find_tile_wave(row_prefix, ntiles, r);
@@ -237,50 +247,47 @@ __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, 
   const int y = static_cast<int>(r - row_prefix[t]);
   const int Y = T.y0 + y;
   const auto rec = gptr(pix + T.px_base + std::uint64_t(y) * T.w);
-  const bool staged = T.w <= kStage;
-  if (staged) {
-    for (int x = threadIdx.x; x < T.w; x += blockDim.x) soff[x] = pix_off(rec[x]);
-    __syncthreads();
-  }
-  auto off_at = [&](int x) { return staged ? soff[x] : pix_off(rec[x]); };
-  const std::uint64_t last = rec[T.w - 1];
-  const std::uint32_t chunks = (pix_off(last) + static_cast<std::uint32_t>(pix_num(last))) >> 4;
   const auto rows = gptr(J.rows4) + 4 * Y;
-  const int bfirst = rows[0], blast = rows[1], pfirst = rows[2], plast = rows[3];
   const auto cl = gptr(J.cl) + std::size_t(Y) * J.cl_stride;
   const auto cr = gptr(J.cr) + std::size_t(Y) * J.cr_stride;
+  const std::uint64_t* bc = bcorr + T.band_base + y;
   const int pw1 = J.pwidth - 1;
-  const std::uint32_t invalid = static_cast<std::uint32_t>(J.invalid_cost);
-  std::uint8_t* out = C + T.cell_base;
   // Partner columns a cost of this row can use: X - dmin - label for X in the
   // tile and label in [0, D); every valid column lies in [c0, c1].
   const int c0 = max(0, T.x0 - J.dmin - (J.D - 1)), c1 = min(pw1, T.x0 + T.w - 1 - J.dmin);
   const bool windowed = c1 >= c0 && c1 - c0 < kWin;
+  const bool staged = T.w <= kPx && T.nbands <= kBands;
+  const std::uint64_t last = rec[T.w - 1];
   if (windowed) {
     for (int k = threadIdx.x; k <= c1 - c0; k += blockDim.x) swin[k] = cr[c0 + k];
   }
+  if (staged) {
+    for (int x = threadIdx.x; x < T.w; x += blockDim.x) {
+      const std::uint64_t pr = rec[x];
+      const std::uint32_t off = pix_off(pr);
+      soff[x] = off;
+      slo[x] = static_cast<std::uint16_t>(pix_lo(pr));
+      scl[x] = cl[T.x0 + x];
+      const std::uint32_t e = min((off + static_cast<std::uint32_t>(pix_num(pr))) >> 4, static_cast<std::uint32_t>(kMap));
+      for (std::uint32_t k = off >> 4; k < e; ++k) smap[k] = static_cast<std::uint16_t>(x);
+    }
+    for (int b = threadIdx.x; b < T.nbands; b += blockDim.x) sbc[b] = bc[std::size_t(b) * T.h];
+  }
   __syncthreads();
+  const std::uint32_t chunks = (pix_off(last) + static_cast<std::uint32_t>(pix_num(last))) >> 4;
+  const int bfirst = rows[0], blast = rows[1], pfirst = rows[2], plast = rows[3];
+  const std::uint32_t invalid = static_cast<std::uint32_t>(J.invalid_cost);
+  std::uint8_t* out = C + T.cell_base;
   auto partner = [&](int xr) {
     return windowed ? swin[min(max(xr, c0), c1) - c0] : cr[min(max(xr, 0), pw1)];
   };
-  const std::uint64_t* bc = bcorr + T.band_base + y;
-  for (std::uint32_t k = threadIdx.x; k < chunks; k += blockDim.x) {
-    // Every pixel holds at least one chunk, so the pixel of chunk k is at most k.
-    int xa = 0, xb = min(static_cast<int>(k), T.w - 1);
-    while (xa < xb) {
-      const int mid = (xa + xb + 1) >> 1;
-      if (off_at(mid) <= 16 * k) {
-        xa = mid;
-      } else {
-        xb = mid - 1;
-      }
-    }
-    const std::uint64_t pr = rec[xa];
+  // Costs of chunk k of pixel xa (first cell off, first label lo, Census c),
+  // stored at the band row offset bco.
+  auto chunk = [&](std::uint32_t k, int xa, std::uint32_t off, int lo, std::uint64_t c, std::uint64_t bco) {
     const int X = T.x0 + xa;
-    const int jb = static_cast<int>(16 * k - pix_off(pr));
-    const int xr0 = X - J.dmin - pix_lo(pr) - jb;
+    const int jb = static_cast<int>(16 * k - off);
+    const int xr0 = X - J.dmin - lo - jb;
     const bool lvalid = X >= bfirst && X <= blast;
-    const std::uint64_t c = cl[X];
     std::uint32_t w[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -295,8 +302,43 @@ __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, 
       }
       w[q] = v;
     }
-    *reinterpret_cast<uint4*>(out + bc[std::size_t(xa >> T.band_shift) * T.h] + 16 * std::size_t(k)) =
-        make_uint4(w[0], w[1], w[2], w[3]);
+    *reinterpret_cast<uint4*>(out + bco + 16 * std::size_t(k)) = make_uint4(w[0], w[1], w[2], w[3]);
+  };
+  if (staged) {
+    const bool mapped = chunks <= static_cast<std::uint32_t>(kMap);
+    for (std::uint32_t k = threadIdx.x; k < chunks; k += blockDim.x) {
+      int xa;
+      if (mapped) {
+        xa = smap[k];
+      } else {
+        // Every pixel holds at least one chunk, so the pixel of chunk k is at most k.
+        xa = 0;
+        int xb = min(static_cast<int>(k), T.w - 1);
+        while (xa < xb) {
+          const int mid = (xa + xb + 1) >> 1;
+          if (soff[mid] <= 16 * k) {
+            xa = mid;
+          } else {
+            xb = mid - 1;
+          }
+        }
+      }
+      chunk(k, xa, soff[xa], slo[xa], scl[xa], sbc[xa >> T.band_shift]);
+    }
+  } else {
+    for (std::uint32_t k = threadIdx.x; k < chunks; k += blockDim.x) {
+      int xa = 0, xb = min(static_cast<int>(k), T.w - 1);
+      while (xa < xb) {
+        const int mid = (xa + xb + 1) >> 1;
+        if (pix_off(rec[mid]) <= 16 * k) {
+          xa = mid;
+        } else {
+          xb = mid - 1;
+        }
+      }
+      const std::uint64_t pr = rec[xa];
+      chunk(k, xa, pix_off(pr), pix_lo(pr), cl[T.x0 + xa], bc[std::size_t(xa >> T.band_shift) * T.h]);
+    }
   }
 }
 
@@ -335,8 +377,8 @@ __device__ __forceinline__ void stb(void* base, std::uint32_t off, T v) {
 // operation and nothing else: vmcnt 0, expcnt 7, lgkmcnt 15.
 constexpr int kVmcnt0 = 0x0F70;
 constexpr int kWaves = 4;                     // waves per block
-constexpr int kDist = 1;                      // steps between a segment prefetch and its use
-constexpr int kNarrow = 256;                  // labels whose path costs stay in LDS
+constexpr int kDist = 2;                      // steps between a segment prefetch and its use
+constexpr int kNarrow = 368;                  // labels whose path costs stay in LDS
 constexpr int kPad = 12;                      // sentinel slots before label 0 and after the last label
 constexpr int kRow = kPad + kNarrow + kPad;   // uint16 slots of an LDS row
 

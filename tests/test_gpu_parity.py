@@ -117,6 +117,17 @@ def test_cost_volume_full_range_512(gpu):
     same_bits(gpu.cost_volume(p.left, p.right, 0, 512), O.cost_volume(p.left, p.right, 0, 512))
 
 
+def test_cost_volume_mixed_wide_ranges(gpu):
+    # Label counts from 16 to 512 in random order along every path: the path
+    # costs of a pixel move between LDS rows (narrow) and scratch rows (wide)
+    # in both directions.
+    w, h, nd = 53, 21, 512
+    left, right, _ = render_pair(Texture(29), w + 40, h, lambda x, y: 60.0 + 0.4 * x - 0.2 * y)
+    lo, num = _ranges(w + 40, h, nd, 41, maxk=32)
+    o = StereoOptions(p2=150, p2_adapt=8)
+    same_bits(gpu.cost_volume(left, right, 3, nd, o, (lo, num)), O.cost_volume(left, right, 3, nd, o, (lo, num)))
+
+
 # ---- ranges and filters
 @pytest.mark.parametrize("mirrored", [False, True])
 @pytest.mark.parametrize("sign", [-1, 1])
