@@ -85,6 +85,9 @@ void launch_sgm_path(int path, const JobDesc* jobs, const TileDesc* tiles, const
                      std::uint32_t* queue, bool accumulate, hipStream_t s);
 // Extra elements S and C hold past their last cell.
 constexpr std::size_t kSPad = 64;
+// dst[i] = (src[0][i] + ... + src[nsrc - 1][i]) mod 2^16 for i < n rounded up
+// to a multiple of 8 (buffers padded by kSPad); dst may be src[0].
+void launch_sum_u16(std::uint16_t* dst, const std::uint16_t* const* src, int nsrc, std::uint64_t n, hipStream_t s);
 // WTA + sub-pixel + checks of every tile row into per-tile-pixel results.
 void launch_select(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* row_prefix, int ntiles,
                    std::uint64_t total_rows, const std::uint64_t* row_start, const std::uint64_t* pix,

@@ -230,10 +230,12 @@ __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, 
                                               const std::uint64_t* __restrict__ row_prefix, int ntiles,
                                               const std::uint64_t* __restrict__ bcorr,
                                               const std::uint64_t* __restrict__ pix, std::uint8_t* __restrict__ C) {
-  constexpr int kPx = 1024;         // row pixels staged in LDS
+  // Sized for the tiles of the default options (at most 600 columns, 512
+  // labels): 24 KB of LDS, six blocks per CU.
+  constexpr int kPx = 640;          // row pixels staged in LDS
   constexpr int kBands = kPx / 8;   // band corrections staged (8-column bands)
-  constexpr int kMap = 4096;        // chunks mapped to their pixel
-  constexpr int kWin = 2048;        // partner Census columns staged
+  constexpr int kMap = 2048;        // chunks mapped to their pixel
+  constexpr int kWin = 1280;        // partner Census columns staged
   __shared__ std::uint32_t soff[kPx];
   __shared__ std::uint16_t slo[kPx];
   __shared__ std::uint64_t scl[kPx];
@@ -744,6 +746,27 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
   }
 }
 
+// ------------------------------------------------------------------ partial sums
+// Sum of the path-cost slices of scans that ran side by side (uint16
+// wrap-around, so the grouping of the 8 paths does not matter).
+struct SumSrc {
+  const std::uint16_t* p[4];
+};
+__global__ __launch_bounds__(256) void k_sum_u16(std::uint16_t* __restrict__ dst, SumSrc src, int nsrc,
+                                                 std::uint64_t n8) {
+  const std::uint64_t i = std::uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n8) return;
+  uint4 acc = reinterpret_cast<const uint4*>(src.p[0])[i];
+  for (int k = 1; k < nsrc; ++k) {
+    const uint4 v = reinterpret_cast<const uint4*>(src.p[k])[i];
+    acc.x = as_u32(as_pair(acc.x) + as_pair(v.x));
+    acc.y = as_u32(as_pair(acc.y) + as_pair(v.y));
+    acc.z = as_u32(as_pair(acc.z) + as_pair(v.z));
+    acc.w = as_u32(as_pair(acc.w) + as_pair(v.w));
+  }
+  reinterpret_cast<uint4*>(dst)[i] = acc;
+}
+
 // ------------------------------------------------------------------ selection
 __device__ __forceinline__ float subpixel_offset(int method, float sm, float s0, float sp) {
   float delta = 0.0F;
@@ -1124,6 +1147,14 @@ void launch_select(const JobDesc* jobs, const TileDesc* tiles, const std::uint64
   const std::size_t lds = p.diagonal_lr ? std::size_t(max_w + max_D) * sizeof(std::uint32_t) : 0;
   k_select<<<dim3(static_cast<unsigned>(total_rows)), 256, lds, s>>>(jobs, tiles, row_prefix, ntiles, row_start,
                                                                       col_off, S, p, tdisp, tconf, tlab);
+}
+
+void launch_sum_u16(std::uint16_t* dst, const std::uint16_t* const* src, int nsrc, std::uint64_t n, hipStream_t s) {
+  const std::uint64_t n8 = (n + 7) / 8;
+  if (n8 == 0 || nsrc <= 0 || nsrc > 4) return;
+  SumSrc ss{};
+  for (int k = 0; k < nsrc; ++k) ss.p[k] = src[k];
+  k_sum_u16<<<dim3(static_cast<unsigned>((n8 + 255) / 256)), 256, 0, s>>>(dst, ss, nsrc, n8);
 }
 
 void launch_pack_rows(const std::int32_t* base, const std::int32_t* partner, int h, std::int32_t* rows4,

@@ -596,8 +596,19 @@ Chunk aggregate_tiles(Context& c, Aggregation& A, int t0, int t1) {
   // Scratch rows of the persistent waves (pixels wider than the LDS rows) and
   // the work-queue counters of the 8 launches.
   // (a multiple of 4: launches use whole blocks of 4 waves, one slot each)
-  const std::uint64_t slots = (std::min(path_slots(), std::max<std::uint64_t>(max_groups, 1)) + 3) / 4 * 4;
-  DBuf<std::uint16_t> scratch(c.arena, path_scratch_elems(slots, max_num));
+  const std::uint64_t resident = path_slots();
+  const std::uint64_t slots = (std::min(resident, std::max<std::uint64_t>(max_groups, 1)) + 3) / 4 * 4;
+  // Scans that fill less than half of the device run side by side on up to 4
+  // streams, each stream summing its paths into a slice of its own, when the
+  // slices and their scratch rows fit the budget.
+  int ways = static_cast<int>(std::clamp<std::uint64_t>(resident / std::max<std::uint64_t>(max_groups, 1), 1, 4));
+  const std::uint64_t slice_bytes =
+      (ch.cells + kSPad) * sizeof(std::uint16_t) + path_scratch_elems(slots, max_num) * sizeof(std::uint16_t);
+  const std::uint64_t free_bytes = c.arena.budget() > c.arena.live() ? c.arena.budget() - c.arena.live() : 0;
+  while (ways > 1 && std::uint64_t(ways - 1) * slice_bytes + (ch.cells + kSPad) > free_bytes / 2) --ways;
+  std::vector<DBuf<std::uint16_t>> slices;
+  for (int w = 1; w < ways; ++w) slices.emplace_back(c.arena, ch.cells + kSPad);
+  DBuf<std::uint16_t> scratch(c.arena, std::size_t(ways) * path_scratch_elems(slots, max_num));
   DBuf<std::uint32_t> queues(c.arena, 8);
   hip_check(hipMemsetAsync(queues.get(), 0, 8 * sizeof(std::uint32_t), c.stream), "hipMemsetAsync");
 
@@ -609,15 +620,40 @@ Chunk aggregate_tiles(Context& c, Aggregation& A, int t0, int t1) {
                 C.get(), c.stream);
     check_launch();
   }
-  // Aggregation: 8 path launches over the chunk's tiles.
+  // Aggregation: 8 path launches over the chunk's tiles; path p runs on
+  // stream p % ways and adds into that stream's slice.
   {
     const StageTimer timer(c, R3DG_STAGE_AGGREGATE);
+    std::array<hipStream_t, 4> st{c.stream, nullptr, nullptr, nullptr};
+    std::array<hipEvent_t, 4> ev{};
+    if (ways > 1) {
+      ev[0] = c.event();
+      hip_check(hipEventRecord(ev[0], c.stream), "hipEventRecord");
+      for (int w = 1; w < ways; ++w) {
+        st[std::size_t(w)] = c.side_stream(w - 1);
+        hip_check(hipStreamWaitEvent(st[std::size_t(w)], ev[0], 0), "hipStreamWaitEvent");
+      }
+    }
     for (int p = 0; p < 8; ++p) {
+      const int w = p % ways;
       const std::size_t o8 = std::size_t(p) * (std::size_t(nt) + 1);
+      std::uint16_t* S = w == 0 ? ch.S.get() : slices[std::size_t(w - 1)].get();
       launch_sgm_path(p, A.djobs.get(), ch.dtiles.get(), dgp.get() + o8, nt, gp[o8 + std::size_t(nt)],
-                      A.bcorr.get(), A.pix.get(), C.get(), ch.S.get(), scratch.get(), slots, max_num,
-                      queues.get() + p, p > 0, c.stream);
+                      A.bcorr.get(), A.pix.get(), C.get(), S, scratch.get() + std::size_t(w) * path_scratch_elems(slots, max_num),
+                      slots, max_num, queues.get() + p, p >= ways, st[std::size_t(w)]);
       check_launch();
+    }
+    if (ways > 1) {
+      std::array<const std::uint16_t*, 4> src{ch.S.get(), nullptr, nullptr, nullptr};
+      for (int w = 1; w < ways; ++w) {
+        ev[std::size_t(w)] = c.event();
+        hip_check(hipEventRecord(ev[std::size_t(w)], st[std::size_t(w)]), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(c.stream, ev[std::size_t(w)], 0), "hipStreamWaitEvent");
+        src[std::size_t(w)] = slices[std::size_t(w - 1)].get();
+      }
+      launch_sum_u16(ch.S.get(), src.data(), ways, ch.cells, c.stream);
+      check_launch();
+      for (int w = 0; w < ways; ++w) c.event_pool.push_back(ev[std::size_t(w)]);
     }
   }
   c.agg_launches += 8;

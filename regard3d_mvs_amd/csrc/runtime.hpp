@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cstdint>
 #include <thread>
@@ -291,8 +292,20 @@ struct Context {
   std::mutex server_mutex;
   std::shared_ptr<void> server;
 
+  // Extra streams for path scans that run side by side (levels whose scans do
+  // not fill the device); created on first use, destroyed with the context.
+  std::array<hipStream_t, 3> side{};
+  hipStream_t side_stream(int i) {
+    if (side[std::size_t(i)] == nullptr)
+      hip_check(hipStreamCreateWithFlags(&side[std::size_t(i)], hipStreamNonBlocking), "hipStreamCreate");
+    return side[std::size_t(i)];
+  }
+
   explicit Context(std::uint64_t budget) : arena(budget) {}
   ~Context() {
+    for (hipStream_t s : side) {
+      if (s != nullptr) (void)hipStreamDestroy(s);
+    }
     for (auto& m : marks) {
       event_pool.push_back(m.start);
       event_pool.push_back(m.stop);

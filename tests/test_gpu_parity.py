@@ -128,6 +128,15 @@ def test_cost_volume_mixed_wide_ranges(gpu):
     same_bits(gpu.cost_volume(left, right, 3, nd, o, (lo, num)), O.cost_volume(left, right, 3, nd, o, (lo, num)))
 
 
+def test_cost_volume_wide_row(gpu):
+    # A single tile wider than the cost pass stages in LDS, with a partner
+    # window wider than its LDS window: the pass reads everything from HBM.
+    w, h, nd = 700, 6, 640
+    left, right, _ = render_pair(Texture(17), w, h, lambda x, y: 200.0 + 0.1 * x)
+    o = StereoOptions(p2=120)
+    same_bits(gpu.cost_volume(left, right, -20, nd, o), O.cost_volume(left, right, -20, nd, o))
+
+
 # ---- ranges and filters
 @pytest.mark.parametrize("mirrored", [False, True])
 @pytest.mark.parametrize("sign", [-1, 1])
@@ -256,6 +265,12 @@ def test_match_chunked_volume(gpu):
     for a, b in ((got.disparity, ref.disparity), (got.confidence, ref.confidence), (got.labels, ref.labels),
                  (got.lr_mask, ref.lr_mask)):
         same_bits(a, b)
+
+
+def test_match_wide_tiles(gpu):
+    # Tiles wider than the selection and cost passes stage in LDS.
+    left, right, _ = render_pair(Texture(23), 1100, 24, lambda x, y: 20.0 + 0.01 * x)
+    _match_both(gpu, left, right, None, None, 0, 48, StereoOptions(tile_size=2048, levels=0))
 
 
 def test_match_hierarchical_wide_range(gpu):
