@@ -582,13 +582,14 @@ Chunk aggregate_tiles(Context& c, Aggregation& A, int t0, int t1) {
   upload(c, ch.drow_prefix, row_prefix);
   // Group prefixes of the 8 paths (kScanPerWave scanlines per wave), one upload.
   std::vector<std::uint64_t> gp(8 * (std::size_t(nt) + 1), 0);
-  std::uint64_t max_groups = 0;
+  std::uint64_t max_groups = 0, sum_groups = 0;
   for (int p = 0; p < 8; ++p) {
     std::uint64_t* g = gp.data() + std::size_t(p) * (std::size_t(nt) + 1);
     for (int t = 0; t < nt; ++t)
       g[t + 1] = g[t] + std::uint64_t((path_scanlines(p, td[std::size_t(t)].w, td[std::size_t(t)].h) +
                                        kScanPerWave - 1) / kScanPerWave);
     max_groups = std::max(max_groups, g[nt]);
+    sum_groups += g[nt];
   }
   DBuf<std::uint64_t> dgp;
   upload(c, dgp, gp);
@@ -598,10 +599,11 @@ Chunk aggregate_tiles(Context& c, Aggregation& A, int t0, int t1) {
   // (a multiple of 4: launches use whole blocks of 4 waves, one slot each)
   const std::uint64_t resident = path_slots();
   const std::uint64_t slots = (std::min(resident, std::max<std::uint64_t>(max_groups, 1)) + 3) / 4 * 4;
-  // Scans that fill less than half of the device run side by side on up to 4
-  // streams, each stream summing its paths into a slice of its own, when the
-  // slices and their scratch rows fit the budget.
-  int ways = static_cast<int>(std::clamp<std::uint64_t>(resident / std::max<std::uint64_t>(max_groups, 1), 1, 4));
+  // Scans whose groups average less than the device holds run side by side on
+  // up to 4 streams, each stream summing its paths into a slice of its own,
+  // when the slices and their scratch rows fit the budget.
+  const std::uint64_t mean_groups = std::max<std::uint64_t>((sum_groups + 7) / 8, 1);
+  int ways = static_cast<int>(std::clamp<std::uint64_t>((resident + mean_groups - 1) / mean_groups, 1, 4));
   const std::uint64_t slice_bytes =
       (ch.cells + kSPad) * sizeof(std::uint16_t) + path_scratch_elems(slots, max_num) * sizeof(std::uint16_t);
   const std::uint64_t free_bytes = c.arena.budget() > c.arena.live() ? c.arena.budget() - c.arena.live() : 0;
