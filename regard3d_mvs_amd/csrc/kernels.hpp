@@ -91,8 +91,8 @@ void launch_sum_u16(std::uint16_t* dst, const std::uint16_t* const* src, int nsr
 // WTA + sub-pixel + checks of every tile row into per-tile-pixel results.
 void launch_select(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* row_prefix, int ntiles,
                    std::uint64_t total_rows, const std::uint64_t* row_start, const std::uint64_t* pix,
-                   const std::uint16_t* S, SelectParams p, int max_w, int max_D, float* tdisp, float* tconf,
-                   std::int16_t* tlab, hipStream_t s);
+                   const std::uint16_t* const* S, int nslices, SelectParams p, int max_w, int max_D, float* tdisp,
+                   float* tconf, std::int16_t* tlab, hipStream_t s);
 // Packs two valid-interval arrays ([h][2] each) into rows4 ([h][4]).
 void launch_pack_rows(const std::int32_t* base, const std::int32_t* partner, int h, std::int32_t* rows4,
                       hipStream_t s);

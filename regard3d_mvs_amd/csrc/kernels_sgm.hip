@@ -830,7 +830,7 @@ __global__ __launch_bounds__(256) void k_select(const JobDesc* __restrict__ jobs
                                                 const std::uint64_t* __restrict__ row_prefix, int ntiles,
                                                 const std::uint64_t* __restrict__ bcorr,
                                                 const std::uint64_t* __restrict__ pix,
-                                                const std::uint16_t* __restrict__ S, SelectParams p,
+                                                SumSrc S, int nslices, SelectParams p,
                                                 float* __restrict__ tdisp, float* __restrict__ tconf,
                                                 std::int16_t* __restrict__ tlab) {
   constexpr int kStage = 1024;  // row pixels whose first cell and range are staged in LDS
@@ -848,7 +848,23 @@ __global__ __launch_bounds__(256) void k_select(const JobDesc* __restrict__ jobs
   const int xb = min(max(lf - T.x0, 0), T.w);
   const int xe = min(max(ll + 1 - T.x0, xb), T.w);
   const std::uint64_t* bc = bcorr + T.band_base + y;
-  const std::uint16_t* St = S + T.cell_base;
+  // Aggregated costs: the sum of the slices of scans that ran side by side.
+  auto ld8 = [&](std::uint64_t c) {
+    uint4 a = *reinterpret_cast<const uint4*>(S.p[0] + c);
+    for (int i = 1; i < nslices; ++i) {
+      const uint4 v = *reinterpret_cast<const uint4*>(S.p[i] + c);
+      a.x = as_u32(as_pair(a.x) + as_pair(v.x));
+      a.y = as_u32(as_pair(a.y) + as_pair(v.y));
+      a.z = as_u32(as_pair(a.z) + as_pair(v.z));
+      a.w = as_u32(as_pair(a.w) + as_pair(v.w));
+    }
+    return a;
+  };
+  auto ld1 = [&](std::uint64_t c) {
+    std::uint32_t a = S.p[0][c];
+    for (int i = 1; i < nslices; ++i) a += S.p[i][c];
+    return a & 0xFFFFU;
+  };
   const std::uint64_t px = T.px_base + std::uint64_t(y) * T.w;
   const int D = J.D;
   const int q = threadIdx.x & 3;
@@ -876,7 +892,7 @@ __global__ __launch_bounds__(256) void k_select(const JobDesc* __restrict__ jobs
     }
     lo = static_cast<int>(range & 0xFFFFU);
     n = static_cast<int>(range >> 16);
-    return St + cell;
+    return T.cell_base + cell;
   };
 
   if (p.diagonal_lr) {
@@ -886,11 +902,11 @@ __global__ __launch_bounds__(256) void k_select(const JobDesc* __restrict__ jobs
     __syncthreads();
     for (int x = xb + (threadIdx.x >> 2); x < xe; x += kPerPass) {
       int lo, num;
-      const std::uint16_t* s = pixel(x, lo, num);
+      const std::uint64_t s = pixel(x, lo, num);
       for (int k0 = 8 * q; k0 < num; k0 += 32) {
 #pragma unroll
         for (int e = 0; e < 8; ++e)
-          atomicMin(&keys[x - (lo + k0 + e) + D - 1], (std::uint32_t(s[k0 + e]) << 16) | std::uint32_t(x));
+          atomicMin(&keys[x - (lo + k0 + e) + D - 1], (ld1(s + k0 + e) << 16) | std::uint32_t(x));
       }
     }
     __syncthreads();
@@ -904,23 +920,23 @@ __global__ __launch_bounds__(256) void k_select(const JobDesc* __restrict__ jobs
     if (x >= xb && x < xe) {
       const int X = T.x0 + x;
       int lo, n;
-      const std::uint16_t* s = pixel(x, lo, n);
+      const std::uint64_t s = pixel(x, lo, n);
       // Labels are a multiple of 16 per pixel, so each lane's group of 8 is
       // either wholly inside the pixel's labels or wholly past them.
       const int k0 = 8 * q, k1 = 32 + 8 * q;
       uint4 g0 = make_uint4(0xFFFFFFFFU, 0xFFFFFFFFU, 0xFFFFFFFFU, 0xFFFFFFFFU), g1 = g0;
-      if (k0 < n) g0 = *reinterpret_cast<const uint4*>(s + k0);
-      if (k1 < n) g1 = *reinterpret_cast<const uint4*>(s + k1);
+      if (k0 < n) g0 = ld8(s + k0);
+      if (k1 < n) g1 = ld8(s + k1);
       std::uint32_t m = 0xFFFFFFFFU;
       if (k0 < n) m = key_min(g0, k0, m);
       if (k1 < n) m = key_min(g1, k1, m);
-      for (int k = 64 + 8 * q; k < n; k += 32) m = key_min(*reinterpret_cast<const uint4*>(s + k), k, m);
+      for (int k = 64 + 8 * q; k < n; k += 32) m = key_min(ld8(s + k), k, m);
       m = quad_min(m);
       const int kb = static_cast<int>(m & 0xFFFFU);
       std::uint32_t second = 0xFFFFU;
       if (k0 < n) second = outside_min(g0, k0, kb, second);
       if (k1 < n) second = outside_min(g1, k1, kb, second);
-      for (int k = 64 + 8 * q; k < n; k += 32) second = outside_min(*reinterpret_cast<const uint4*>(s + k), k, kb, second);
+      for (int k = 64 + 8 * q; k < n; k += 32) second = outside_min(ld8(s + k), k, kb, second);
       second = quad_min(second);
       // Costs at kb - 1 and kb + 1 from the registers of the quad when they
       // lie in the first 64 labels.
@@ -945,8 +961,8 @@ __global__ __launch_bounds__(256) void k_select(const JobDesc* __restrict__ jobs
       if (ok) {
         float delta = 0.0F;
         if (j > 0 && j < n - 1) {
-          const std::uint32_t vm = j - 1 < 64 ? sm : s[j - 1];
-          const std::uint32_t vp = j + 1 < 64 ? sp : s[j + 1];
+          const std::uint32_t vm = j - 1 < 64 ? sm : ld1(s + j - 1);
+          const std::uint32_t vp = j + 1 < 64 ? sp : ld1(s + j + 1);
           delta = subpixel_offset(p.subpixel, static_cast<float>(vm), best_cost, static_cast<float>(vp));
         }
         d = static_cast<float>(J.dmin + best) + delta;
@@ -1142,11 +1158,14 @@ void launch_sgm_path(int path, const JobDesc* jobs, const TileDesc* tiles, const
 
 void launch_select(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* row_prefix, int ntiles,
                    std::uint64_t total_rows, const std::uint64_t* row_start, const std::uint64_t* col_off,
-                   const std::uint16_t* S, SelectParams p, int max_w, int max_D, float* tdisp, float* tconf,
-                   std::int16_t* tlab, hipStream_t s) {
+                   const std::uint16_t* const* S, int nslices, SelectParams p, int max_w, int max_D, float* tdisp,
+                   float* tconf, std::int16_t* tlab, hipStream_t s) {
+  if (total_rows == 0 || nslices < 1 || nslices > 4) return;
+  SumSrc ss{};
+  for (int k = 0; k < nslices; ++k) ss.p[k] = S[k];
   const std::size_t lds = p.diagonal_lr ? std::size_t(max_w + max_D) * sizeof(std::uint32_t) : 0;
   k_select<<<dim3(static_cast<unsigned>(total_rows)), 256, lds, s>>>(jobs, tiles, row_prefix, ntiles, row_start,
-                                                                      col_off, S, p, tdisp, tconf, tlab);
+                                                                      col_off, ss, nslices, p, tdisp, tconf, tlab);
 }
 
 void launch_sum_u16(std::uint16_t* dst, const std::uint16_t* const* src, int nsrc, std::uint64_t n, hipStream_t s) {

@@ -406,7 +406,15 @@ struct Chunk {
   DBuf<TileDesc> dtiles;
   DBuf<std::uint64_t> drow_prefix;
   std::uint64_t rows = 0, cells = 0, px = 0;
+  // Aggregated costs: S plus the slices of path scans that ran side by side
+  // on other streams (the sum mod 2^16 of all of them).
   DBuf<std::uint16_t> S;
+  std::vector<DBuf<std::uint16_t>> slices;
+  std::vector<const std::uint16_t*> parts() const {
+    std::vector<const std::uint16_t*> v{S.get()};
+    for (const auto& b : slices) v.push_back(b.get());
+    return v;
+  }
 };
 
 // band_shift: log2 of the volume band width (TileDesc); 31 stores every tile
@@ -560,7 +568,8 @@ Aggregation layout_jobs(Context& c, std::vector<Job>& jobs, const r3dg_options& 
 }
 
 // Costs and the 8 path scans of tiles [t0, t1) (their rows are consecutive).
-Chunk aggregate_tiles(Context& c, Aggregation& A, int t0, int t1) {
+// With `sum_slices` the slices of side-by-side scans are added into S.
+Chunk aggregate_tiles(Context& c, Aggregation& A, int t0, int t1, bool sum_slices) {
   Chunk ch;
   ch.t0 = t0;
   ch.t1 = t1;
@@ -599,16 +608,17 @@ Chunk aggregate_tiles(Context& c, Aggregation& A, int t0, int t1) {
   // (a multiple of 4: launches use whole blocks of 4 waves, one slot each)
   const std::uint64_t resident = path_slots();
   const std::uint64_t slots = (std::min(resident, std::max<std::uint64_t>(max_groups, 1)) + 3) / 4 * 4;
-  // Scans whose groups average less than the device holds run side by side on
-  // up to 4 streams, each stream summing its paths into a slice of its own,
-  // when the slices and their scratch rows fit the budget.
+  // The scans run side by side on 2 to 4 streams (more where their groups
+  // average less than the device holds), each stream summing its paths into a
+  // slice of its own, when the slices and their scratch rows fit the budget:
+  // a launch's last groups then overlap the next launch on another stream.
   const std::uint64_t mean_groups = std::max<std::uint64_t>((sum_groups + 7) / 8, 1);
-  int ways = static_cast<int>(std::clamp<std::uint64_t>((resident + mean_groups - 1) / mean_groups, 1, 4));
+  int ways = static_cast<int>(std::clamp<std::uint64_t>((resident + mean_groups - 1) / mean_groups, 2, 4));
   const std::uint64_t slice_bytes =
       (ch.cells + kSPad) * sizeof(std::uint16_t) + path_scratch_elems(slots, max_num) * sizeof(std::uint16_t);
   const std::uint64_t free_bytes = c.arena.budget() > c.arena.live() ? c.arena.budget() - c.arena.live() : 0;
   while (ways > 1 && std::uint64_t(ways - 1) * slice_bytes + (ch.cells + kSPad) > free_bytes / 2) --ways;
-  std::vector<DBuf<std::uint16_t>> slices;
+  std::vector<DBuf<std::uint16_t>>& slices = ch.slices;
   for (int w = 1; w < ways; ++w) slices.emplace_back(c.arena, ch.cells + kSPad);
   DBuf<std::uint16_t> scratch(c.arena, std::size_t(ways) * path_scratch_elems(slots, max_num));
   DBuf<std::uint32_t> queues(c.arena, 8);
@@ -646,15 +656,17 @@ Chunk aggregate_tiles(Context& c, Aggregation& A, int t0, int t1) {
       check_launch();
     }
     if (ways > 1) {
-      std::array<const std::uint16_t*, 4> src{ch.S.get(), nullptr, nullptr, nullptr};
       for (int w = 1; w < ways; ++w) {
         ev[std::size_t(w)] = c.event();
         hip_check(hipEventRecord(ev[std::size_t(w)], st[std::size_t(w)]), "hipEventRecord");
         hip_check(hipStreamWaitEvent(c.stream, ev[std::size_t(w)], 0), "hipStreamWaitEvent");
-        src[std::size_t(w)] = slices[std::size_t(w - 1)].get();
       }
-      launch_sum_u16(ch.S.get(), src.data(), ways, ch.cells, c.stream);
-      check_launch();
+      if (sum_slices) {
+        const std::vector<const std::uint16_t*> src = ch.parts();
+        launch_sum_u16(ch.S.get(), src.data(), ways, ch.cells, c.stream);
+        check_launch();
+        slices.clear();
+      }
       for (int w = 0; w < ways; ++w) c.event_pool.push_back(ev[std::size_t(w)]);
     }
   }
@@ -700,11 +712,13 @@ std::vector<TilesOut> match_tiles(Context& c, std::vector<Job>& jobs, const r3dg
   DBuf<std::int16_t> tlab;
   if (any_labels) tlab = DBuf<std::int16_t>(c.arena, std::max<std::uint64_t>(total_px, 1));
   for (const auto& [t0, t1] : chunk_tiles(c, A)) {
-    Chunk ch = aggregate_tiles(c, A, t0, t1);
+    Chunk ch = aggregate_tiles(c, A, t0, t1, false);
     // Selection into per-tile maps.
     const StageTimer timer(c, R3DG_STAGE_SELECT);
+    const std::vector<const std::uint16_t*> parts = ch.parts();
     launch_select(A.djobs.get(), ch.dtiles.get(), ch.drow_prefix.get(), t1 - t0, ch.rows, A.bcorr.get(),
-                  A.pix.get(), ch.S.get(), sp, A.max_w, A.max_D, tdisp.get(), tconf.get(), tlab.get(), c.stream);
+                  A.pix.get(), parts.data(), static_cast<int>(parts.size()), sp, A.max_w, A.max_D, tdisp.get(),
+                  tconf.get(), tlab.get(), c.stream);
     check_launch();
   }
   std::vector<TilesOut> out(static_cast<std::size_t>(nj));
@@ -1527,7 +1541,7 @@ int32_t r3dg_cost_volume(r3dg_ctx* h, r3dg_image left, r3dg_image right, int32_t
     jobs[0].D = D;
     Aggregation A = layout_jobs(c, jobs, one, 31);  // rows contiguous: the reference's packed order
     if (A.total_cells != out_cells) fail(R3DG_ERROR_INTERNAL, "cost_volume: cell count mismatch");
-    const Chunk ch = aggregate_tiles(c, A, 0, static_cast<int>(A.td.size()));
+    const Chunk ch = aggregate_tiles(c, A, 0, static_cast<int>(A.td.size()), true);
     const hipMemcpyKind kind = out_memory == R3DG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
     if (out_cells > 0) hip_check(hipMemcpyAsync(out, ch.S.get(), out_cells * 2, kind, c.stream), "download");
     sync(c);
