@@ -108,7 +108,8 @@ typedef struct r3dg_match_info {
   uint64_t tile_pixels;    /* sum of tile pixels over levels and directions */
   double seconds;          /* wall time of the call */
   uint64_t aggregate_launches; /* path-kernel launches */
-  uint64_t aggregate_bytes;    /* algorithmic bytes of the path kernels: sum of 38 C + 80 P */
+  uint64_t aggregate_bytes;    /* algorithmic bytes of the path kernels: sum of (40 - 2 w) C + 80 P,
+                                  w the streams the 8 scans of a chunk ran on (38 C + 80 P for one) */
   double stage_seconds[R3DG_STAGES]; /* device time per stage (0 unless profiling) */
 } r3dg_match_info;
 

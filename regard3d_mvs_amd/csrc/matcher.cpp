@@ -672,8 +672,9 @@ Chunk aggregate_tiles(Context& c, Aggregation& A, int t0, int t1, bool sum_slice
   }
   c.agg_launches += 8;
   // Algorithmic bytes of the 8 launches: per cell C read 8 x 1, S written
-  // 8 x 2 and read 7 x 2; per tile pixel and path its record (8) and intensity (2).
-  c.agg_bytes += 38 * ch.cells + 80 * ch.px;
+  // 8 x 2 and read 8 - ways times x 2 (the first scan of each stream only
+  // writes its slice); per tile pixel and path its record (8) and intensity (2).
+  c.agg_bytes += std::uint64_t(40 - 2 * ways) * ch.cells + 80 * ch.px;
   return ch;
 }
 
