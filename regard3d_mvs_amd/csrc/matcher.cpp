@@ -1614,9 +1614,14 @@ int32_t r3dg_match_batch(r3dg_ctx* h, r3dg_pair* pairs, int32_t count, const r3d
     std::vector<std::unique_ptr<Inputs>> in(static_cast<std::size_t>(count));
     std::vector<int> ok;
     for (int i = 0; i < count; ++i) {
+      pairs[i].status = R3DG_OK;
+      std::memset(&pairs[i].info, 0, sizeof(pairs[i].info));
+      ok.push_back(i);
+    }
+    // A pair's images are uploaded (and checked) when its group is matched,
+    // so that host staging memory is bounded by one group.
+    auto load = [&](int i) -> bool {
       r3dg_pair& q = pairs[i];
-      q.status = R3DG_OK;
-      std::memset(&q.info, 0, sizeof(q.info));
       try {
         auto x = std::make_unique<Inputs>();
         x->L.img = input_img<std::uint16_t>(c, q.left, "left");
@@ -1632,15 +1637,16 @@ int32_t r3dg_match_batch(r3dg_ctx* h, r3dg_pair* pairs, int32_t count, const r3d
         P.dmin = q.disparity_min;
         P.nd = q.num_disparities;
         in[std::size_t(i)] = std::move(x);
-        ok.push_back(i);
+        return true;
       } catch (const Failure& e) {
         q.status = e.status;
         if (first_error == R3DG_OK) {
           first_error = e.status;
           first_message = e.what();
         }
+        return false;
       }
-    }
+    };
     c.agg_launches = 0;
     c.agg_bytes = 0;
     double stage_seconds[R3DG_STAGES] = {};
@@ -1666,10 +1672,14 @@ int32_t r3dg_match_batch(r3dg_ctx* h, r3dg_pair* pairs, int32_t count, const r3d
         ++g1;
       }
       std::vector<PairRun*> group;
-      for (std::size_t k = g0; k < g1; ++k) group.push_back(&in[std::size_t(ok[k])]->run);
-      match_batch(c, group, o);
+      std::vector<int> members;
       for (std::size_t k = g0; k < g1; ++k) {
-        const int i = ok[k];
+        if (!load(ok[k])) continue;
+        group.push_back(&in[std::size_t(ok[k])]->run);
+        members.push_back(ok[k]);
+      }
+      if (!group.empty()) match_batch(c, group, o);
+      for (const int i : members) {
         r3dg_pair& q = pairs[i];
         Inputs& x = *in[std::size_t(i)];
         const TilesOut& m = x.run.res.final.left;
@@ -1689,6 +1699,7 @@ int32_t r3dg_match_batch(r3dg_ctx* h, r3dg_pair* pairs, int32_t count, const r3d
     c.collect(stage_seconds, R3DG_STAGES);
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     for (const int i : ok) {
+      if (pairs[i].status != R3DG_OK) continue;
       pairs[i].info.seconds = secs;
       pairs[i].info.aggregate_launches = c.agg_launches;
       pairs[i].info.aggregate_bytes = c.agg_bytes;
