@@ -251,6 +251,48 @@ int32_t r3dg_ranges_from_coarse(r3dg_ctx* ctx, r3dg_image coarse, int32_t width,
 /* detail::automatic_levels (stereo_matcher.hpp:128). Host-only arithmetic. */
 int32_t r3dg_automatic_levels(int32_t width, int32_t height, int32_t num_disparities);
 
+/* ---- rectification (the step before the matcher) ------------------------- */
+
+/* Camera models of core/camera_model.hpp:13-20 (OpenMVG formulas; params in
+ * that order: radial_k1 k1; radial_k3 k1 k2 k3; brown_t2 k1 k2 k3 t1 t2;
+ * fisheye k1..k4). */
+typedef enum r3dg_camera_model {
+  R3DG_CAMERA_PINHOLE = 0,
+  R3DG_CAMERA_RADIAL_K1 = 1,
+  R3DG_CAMERA_RADIAL_K3 = 2,
+  R3DG_CAMERA_BROWN_T2 = 3,
+  R3DG_CAMERA_FISHEYE = 4
+} r3dg_camera_model;
+
+/* CameraIntrinsics (core/camera_model.hpp:32-41) with its Pose
+ * (core/camera.hpp:9-12): rotation row-major, world to camera axes; centre
+ * in world coordinates.  A PinholeCamera (core/camera.hpp:58-61) is the same
+ * record with model R3DG_CAMERA_PINHOLE. */
+typedef struct r3dg_camera {
+  int32_t model; /* r3dg_camera_model */
+  int32_t width;
+  int32_t height;
+  int32_t reserved;
+  double fx, fy, cx, cy;
+  double params[5];
+  double rotation[9];
+  double center[3];
+} r3dg_camera;
+
+/* Element types of r3dg_rectify. */
+typedef enum r3dg_dtype { R3DG_U8 = 0, R3DG_U16 = 1, R3DG_F32 = 2 } r3dg_dtype;
+
+/* rectify_image<T> (rectify/rectification.hpp:80-82, .cpp:288-350), one
+ * channel: resamples `src`, an image of `camera`, into the pinhole camera
+ * `target` of the same centre; bicubic (Catmull-Rom, core/sampling.hpp:21-55)
+ * from the pyramid level whose resolution is closest above 1/1.5 of the
+ * target's; out-of-image pixels are 0.  out: target width x height of dtype;
+ * valid (uint8, may be empty) receives 255 inside and 0 outside.  Errors as
+ * the reference: a source whose size differs from the camera's, or a target
+ * with another centre, is R3DG_ERROR_INVALID_ARGUMENT. */
+int32_t r3dg_rectify(r3dg_ctx* ctx, int32_t dtype, r3dg_image src, const r3dg_camera* camera,
+                     const r3dg_camera* target, r3dg_image out, r3dg_image valid);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

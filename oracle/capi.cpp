@@ -33,6 +33,34 @@ oracle::Ranges ranges_in(const std::uint16_t* lo, const std::uint16_t* num, int 
 }
 }  // namespace
 
+static oracle::Camera camera_of(const r3dg_camera* c) {
+  oracle::Camera o;
+  o.model = c->model;
+  o.width = c->width;
+  o.height = c->height;
+  o.fx = c->fx;
+  o.fy = c->fy;
+  o.cx = c->cx;
+  o.cy = c->cy;
+  std::memcpy(o.params, c->params, sizeof(o.params));
+  std::memcpy(o.rotation, c->rotation, sizeof(o.rotation));
+  std::memcpy(o.center, c->center, sizeof(o.center));
+  return o;
+}
+
+template <typename T>
+static int rectify_impl(const T* src, int w, int h, const r3dg_camera* cam, const r3dg_camera* target, T* out,
+                        std::uint8_t* valid) {
+  return guarded([&] {
+    const oracle::Camera c = camera_of(cam), t = camera_of(target);
+    const auto s = copy_in(src, w, h);
+    oracle::ImgU8 v;
+    const auto r = oracle::rectify_image(s, c, t, &v);
+    std::memcpy(out, r.px.data(), sizeof(T) * r.px.size());
+    if (valid != nullptr) std::memcpy(valid, v.px.data(), v.px.size());
+  });
+}
+
 extern "C" {
 
 const char* oracle_last_error(void) { return g_error.c_str(); }
@@ -149,6 +177,19 @@ int oracle_match(const std::uint16_t* left, int wl, int h, const std::uint16_t* 
       info->seconds = 0.0;
     }
   });
+}
+
+int oracle_rectify_u8(const std::uint8_t* src, int w, int h, const r3dg_camera* cam, const r3dg_camera* target, std::uint8_t* out,
+                      std::uint8_t* valid) {
+  return rectify_impl(src, w, h, cam, target, out, valid);
+}
+int oracle_rectify_u16(const std::uint16_t* src, int w, int h, const r3dg_camera* cam, const r3dg_camera* target,
+                       std::uint16_t* out, std::uint8_t* valid) {
+  return rectify_impl(src, w, h, cam, target, out, valid);
+}
+int oracle_rectify_f32(const float* src, int w, int h, const r3dg_camera* cam, const r3dg_camera* target, float* out,
+                       std::uint8_t* valid) {
+  return rectify_impl(src, w, h, cam, target, out, valid);
 }
 
 }  // extern "C"

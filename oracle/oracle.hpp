@@ -17,6 +17,7 @@
 //   remove_speckles   KATs of test_stereo.cpp:451-469 + connected components
 //   median_filter_3x3 KATs of test_stereo.cpp:451-469 + sorted-window median
 //   match_stereo      behaviour thresholds of test_stereo.cpp:191-449
+//   rectify_image     see rectify.cpp
 // Floating-point steps (sub-pixel fit, blending, LR rounding) are restated
 // operation by operation and compiled with -ffp-contract=off.
 #pragma once
@@ -72,6 +73,22 @@ ImgU64 census(const ImgU16& img);
 ImgU16 pyramid_down(const ImgU16& img);
 ImgU8 pyramid_down(const ImgU8& img);
 ImgU8 reduce_mask(const ImgU8& mask);
+
+// A camera of rectify_image: CameraIntrinsics + Pose (core/camera_model.hpp:
+// 32-41, core/camera.hpp:9-12); model 0..4 as r3dg_camera_model.
+struct Camera {
+  int model = 0;
+  int width = 0;
+  int height = 0;
+  double fx = 0.0, fy = 0.0, cx = 0.0, cy = 0.0;
+  double params[5] = {};
+  double rotation[9] = {};
+  double center[3] = {};
+};
+
+// rectify_image (rectification.cpp:288-350), one channel (rectify.cpp).
+template <typename T>
+Img<T> rectify_image(const Img<T>& src, const Camera& cam, const Camera& target, ImgU8* valid);
 
 int automatic_levels(int width, int height, int num_disparities);
 

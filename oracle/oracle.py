@@ -65,6 +65,18 @@ def pyramid_down(img):
     return out
 
 
+def rectify(src, camera, target):
+    """rectify_image (rectification.cpp:288-350), one channel: (image, valid)."""
+    src = np.ascontiguousarray(src)
+    fn = {np.dtype(np.uint8): lib().oracle_rectify_u8, np.dtype(np.uint16): lib().oracle_rectify_u16,
+          np.dtype(np.float32): lib().oracle_rectify_f32}[src.dtype]
+    out = np.empty((target.height, target.width), src.dtype)
+    valid = np.empty((target.height, target.width), np.uint8)
+    c, t = camera.to_c(), target.to_c()
+    _check(fn(_ptr(src), src.shape[1], src.shape[0], ctypes.byref(c), ctypes.byref(t), _ptr(out), _ptr(valid)))
+    return out, valid
+
+
 def reduce_mask(mask):
     mask = _c(mask, np.uint8)
     h, w = mask.shape

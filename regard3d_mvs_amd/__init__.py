@@ -1,2 +1,2 @@
 """MI355X-native Census-SGM stereo matcher (drop-in for r3d::match_stereo)."""
-from ._abi import DisparityMap, StereoOptions, SubpixelMethod, align16  # noqa: F401
+from ._abi import Camera, CameraModel, DisparityMap, StereoOptions, SubpixelMethod, align16  # noqa: F401

@@ -46,6 +46,58 @@ class COptions(ctypes.Structure):
                 ("tile_blend", ctypes.c_int32), ("memory_budget", ctypes.c_uint64)]
 
 
+class CameraModel(enum.IntEnum):
+    """r3d::CameraModel (core/camera_model.hpp:13-20)."""
+    pinhole = 0
+    radial_k1 = 1
+    radial_k3 = 2
+    brown_t2 = 3
+    fisheye = 4
+
+
+class CCamera(ctypes.Structure):
+    _fields_ = [("model", ctypes.c_int32), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("fx", ctypes.c_double), ("fy", ctypes.c_double),
+                ("cx", ctypes.c_double), ("cy", ctypes.c_double), ("params", ctypes.c_double * 5),
+                ("rotation", ctypes.c_double * 9), ("center", ctypes.c_double * 3)]
+
+
+@dataclasses.dataclass
+class Camera:
+    """CameraIntrinsics with its Pose (core/camera_model.hpp:32-41, core/camera.hpp:9-12):
+    rotation maps world to camera axes (row-major 3x3), centre in world
+    coordinates.  A PinholeCamera is a Camera of model pinhole."""
+    width: int
+    height: int
+    fx: float
+    fy: float
+    cx: float
+    cy: float
+    model: CameraModel = CameraModel.pinhole
+    params: tuple = (0.0, 0.0, 0.0, 0.0, 0.0)
+    rotation: tuple = (1.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0)
+    center: tuple = (0.0, 0.0, 0.0)
+
+    def to_c(self) -> CCamera:
+        def flat(v, n):
+            out = []
+            for e in v:
+                if hasattr(e, "__len__"):
+                    out.extend(float(x) for x in e)
+                else:
+                    out.append(float(e))
+            if len(out) != n:
+                raise ValueError(f"expected {n} values, got {len(out)}")
+            return out
+        params = [float(v) for v in self.params]
+        if len(params) > 5:
+            raise ValueError("at most 5 distortion parameters")
+        params += [0.0] * (5 - len(params))
+        return CCamera(int(self.model), int(self.width), int(self.height), 0, float(self.fx), float(self.fy),
+                       float(self.cx), float(self.cy), (ctypes.c_double * 5)(*params),
+                       (ctypes.c_double * 9)(*flat(self.rotation, 9)), (ctypes.c_double * 3)(*flat(self.center, 3)))
+
+
 STAGES = ("pyramid", "prepare", "ranges", "layout", "aggregate", "select", "blend", "filter")
 
 

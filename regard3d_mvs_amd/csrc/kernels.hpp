@@ -25,6 +25,28 @@ void launch_mirror_u8(const std::uint8_t* in, std::int64_t is, int w, int h, std
 // rows[y] = {first, last} valid column (mask >= 128); mask == nullptr: {0, w - 1}.
 void launch_valid_intervals(const std::uint8_t* mask, std::int64_t ms, int w, int h, std::int32_t* rows,
                             hipStream_t s);
+// pyramid_down<float> (pyramid.cpp:34-85): float sums, * (1 / 256).
+void launch_pyramid_f32(const float* in, std::int64_t is, int w, int h, float* tmp, float* out, std::int64_t os,
+                        hipStream_t s);
+// Geometry of one rectify_image call (rectification.cpp:288-350): the target
+// pinhole intrinsics, the target-to-source rotation m (row-major) and the
+// source camera at the sampled pyramid level.
+struct RectifyGeom {
+  double m[9];
+  double tfx, tfy, tcx, tcy;
+  double fx, fy, cx, cy;
+  double params[5];
+  int tw, th;
+  int sw, sh;
+  int model;
+};
+// out (tw x th) and valid (may be null) from the source level image src.
+void launch_rectify_u8(const std::uint8_t* src, std::int64_t ss, const RectifyGeom& g, std::uint8_t* out,
+                       std::int64_t os, std::uint8_t* valid, std::int64_t vs, hipStream_t s);
+void launch_rectify_u16(const std::uint16_t* src, std::int64_t ss, const RectifyGeom& g, std::uint16_t* out,
+                        std::int64_t os, std::uint8_t* valid, std::int64_t vs, hipStream_t s);
+void launch_rectify_f32(const float* src, std::int64_t ss, const RectifyGeom& g, float* out, std::int64_t os,
+                        std::uint8_t* valid, std::int64_t vs, hipStream_t s);
 void launch_fill_f32(float* p, std::size_t n, float v, hipStream_t s);
 void launch_fill_u8(std::uint8_t* p, std::size_t n, std::uint8_t v, hipStream_t s);
 

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <limits>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -88,6 +90,28 @@ __global__ void k_pyr_v(const std::uint32_t* __restrict__ tmp, int h, int dw, in
   std::uint32_t v = (s + 128U) >> 8U;
   if (threshold) v = v >= 128U ? 255U : 0U;  // reduce_mask, stereo_matcher.cpp:892-900
   out[y * os + x] = static_cast<T>(v);
+}
+
+// pyramid.cpp:34-85 for float images: float sums, scaled by 1/256.
+__global__ void k_pyr_hf(const float* __restrict__ in, std::int64_t is, int w, int dw, float* __restrict__ tmp) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= dw) return;
+  const float* r = in + y * is;
+  const int sx = 2 * x;
+  const float a = r[max(sx - 2, 0)], b = r[max(sx - 1, 0)], c = r[sx], d = r[min(sx + 1, w - 1)],
+              e = r[min(sx + 2, w - 1)];
+  tmp[std::int64_t(y) * dw + x] = a + e + 4.0F * (b + d) + 6.0F * c;
+}
+
+__global__ void k_pyr_vf(const float* __restrict__ tmp, int h, int dw, int dh, float* __restrict__ out,
+                         std::int64_t os) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= dw) return;
+  const int sy = 2 * y;
+  auto at = [&](int yy) { return tmp[std::int64_t(yy) * dw + x]; };
+  const float s = at(max(sy - 2, 0)) + at(min(sy + 2, h - 1)) + 4.0F * (at(max(sy - 1, 0)) + at(min(sy + 1, h - 1))) +
+                  6.0F * at(sy);
+  out[y * os + x] = s * (1.0F / 256.0F);
 }
 
 template <typename T>
@@ -364,6 +388,122 @@ __global__ void k_median3x3(const float* __restrict__ in, int w, int h, float* _
   out[std::int64_t(y) * w + x] = m;
 }
 
+// ------------------------------------------------------------------ rectification
+// rectify_image (rectify/rectification.cpp:288-350), one thread per target
+// pixel: the ray (x - cx) / fx, (y - cy) / fy, 1 of the target camera turned
+// into the source camera, projected with its lens distortion
+// (camera_model.cpp:68-103, OpenMVG formulas), and sampled bicubically
+// (sampling.hpp:21-55).  Coordinates in double, weights and sums in float, in
+// the reference's order of operations (the library is built without FMA
+// contraction).
+__device__ __forceinline__ void distort(const RectifyGeom& g, double ux, double uy, double& dx, double& dy) {
+  const double r2 = ux * ux + uy * uy;
+  const double* k = g.params;
+  switch (g.model) {
+    case 1: {
+      const double f = 1.0 + k[0] * r2;
+      dx = ux * f;
+      dy = uy * f;
+      return;
+    }
+    case 2: {
+      const double f = 1.0 + r2 * (k[0] + r2 * (k[1] + r2 * k[2]));
+      dx = ux * f;
+      dy = uy * f;
+      return;
+    }
+    case 3: {
+      const double radial = r2 * (k[0] + r2 * (k[1] + r2 * k[2]));
+      const double tx = k[4] * (r2 + 2.0 * ux * ux) + 2.0 * k[3] * ux * uy;
+      const double ty = k[3] * (r2 + 2.0 * uy * uy) + 2.0 * k[4] * ux * uy;
+      dx = ux + ux * radial + tx;
+      dy = uy + uy * radial + ty;
+      return;
+    }
+    case 4: {
+      const double r = sqrt(r2);
+      if (r <= 1e-8) break;
+      const double theta = atan(r);
+      const double t2 = theta * theta;
+      const double theta_d = theta * (1.0 + t2 * (k[0] + t2 * (k[1] + t2 * (k[2] + t2 * k[3]))));
+      const double f = theta_d / r;
+      dx = ux * f;
+      dy = uy * f;
+      return;
+    }
+    default:
+      break;
+  }
+  dx = ux;
+  dy = uy;
+}
+
+// Catmull-Rom weights of the taps -1, 0, 1, 2 around floor(x) (sampling.hpp:21-28).
+__device__ __forceinline__ void cubic_weights(float t, float* w) {
+  const float t2 = t * t;
+  const float t3 = t2 * t;
+  w[0] = -0.5F * t3 + t2 - 0.5F * t;
+  w[1] = 1.5F * t3 - 2.5F * t2 + 1.0F;
+  w[2] = -1.5F * t3 + 2.0F * t2 + 0.5F * t;
+  w[3] = 0.5F * t3 - 0.5F * t2;
+}
+
+template <typename T>
+__device__ __forceinline__ T to_pixel(float v) {
+  if constexpr (std::is_floating_point_v<T>) {
+    return static_cast<T>(v);
+  } else {
+    const float r = roundf(v);
+    const float hi = static_cast<float>(std::numeric_limits<T>::max());
+    return static_cast<T>(r < 0.0F ? 0.0F : (hi < r ? hi : r));
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_rectify(const T* __restrict__ src, std::int64_t ss, RectifyGeom g,
+                                                 T* __restrict__ out, std::int64_t os,
+                                                 std::uint8_t* __restrict__ valid, std::int64_t vs) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  if (x >= g.tw) return;
+  const double ry = (y - g.tcy) / g.tfy;
+  const double vx = (x - g.tcx) / g.tfx;
+  const double r0 = g.m[0] * vx + g.m[1] * ry + g.m[2] * 1.0;
+  const double r1 = g.m[3] * vx + g.m[4] * ry + g.m[5] * 1.0;
+  const double r2 = g.m[6] * vx + g.m[7] * ry + g.m[8] * 1.0;
+  bool inside = r2 > 1e-9 && g.sw > 0 && g.sh > 0;  // an empty source has no pixel inside
+  double sx = 0.0, sy = 0.0;
+  if (inside) {
+    double dx, dy;
+    distort(g, r0 / r2, r1 / r2, dx, dy);
+    sx = g.fx * dx + g.cx;
+    sy = g.fy * dy + g.cy;
+    inside = sx >= -0.5 && sy >= -0.5 && sx <= g.sw - 0.5 && sy <= g.sh - 0.5;
+  }
+  T v = T{};
+  if (inside) {
+    const double fxl = floor(sx), fyl = floor(sy);
+    float wx[4], wy[4];
+    cubic_weights(static_cast<float>(sx - fxl), wx);
+    cubic_weights(static_cast<float>(sy - fyl), wy);
+    const int x0 = static_cast<int>(fxl) - 1, y0 = static_cast<int>(fyl) - 1;
+    int xs[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) xs[i] = min(max(x0 + i, 0), g.sw - 1);
+    float acc_out = 0.0F;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const T* row = src + std::int64_t(min(max(y0 + j, 0), g.sh - 1)) * ss;
+      float acc = 0.0F;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) acc += wx[i] * static_cast<float>(row[xs[i]]);
+      acc_out += wy[j] * acc;
+    }
+    v = to_pixel<T>(acc_out);
+  }
+  out[std::int64_t(y) * os + x] = v;
+  if (valid != nullptr) valid[std::int64_t(y) * vs + x] = inside ? 255 : 0;
+}
+
 // ------------------------------------------------------------------ label ranges
 // ranges_from_coarse (stereo_matcher.cpp:1124-1267).
 __global__ void k_rng_h(const float* __restrict__ coarse, int wc, int hc, int r, float* __restrict__ hmin,
@@ -502,6 +642,35 @@ void launch_pyramid_u8(const std::uint8_t* in, std::int64_t is, int w, int h, st
   const int dw = (w + 1) / 2, dh = (h + 1) / 2;
   k_pyr_h<std::uint8_t><<<dim3((dw + 255) / 256, h), 256, 0, s>>>(in, is, w, h, dw, tmp);
   k_pyr_v<std::uint8_t><<<dim3((dw + 255) / 256, dh), 256, 0, s>>>(tmp, h, dw, dh, out, os, threshold ? 1 : 0);
+}
+
+void launch_pyramid_f32(const float* in, std::int64_t is, int w, int h, float* tmp, float* out, std::int64_t os,
+                        hipStream_t s) {
+  const int dw = (w + 1) / 2, dh = (h + 1) / 2;
+  k_pyr_hf<<<dim3((dw + 255) / 256, h), 256, 0, s>>>(in, is, w, dw, tmp);
+  k_pyr_vf<<<dim3((dw + 255) / 256, dh), 256, 0, s>>>(tmp, h, dw, dh, out, os);
+}
+
+template <typename T>
+void rectify_launch(const T* src, std::int64_t ss, const RectifyGeom& g, T* out, std::int64_t os,
+                    std::uint8_t* valid, std::int64_t vs, hipStream_t s) {
+  if (g.tw <= 0 || g.th <= 0) return;
+  k_rectify<T><<<dim3((g.tw + 255) / 256, g.th), 256, 0, s>>>(src, ss, g, out, os, valid, vs);
+}
+
+void launch_rectify_u8(const std::uint8_t* src, std::int64_t ss, const RectifyGeom& g, std::uint8_t* out,
+                       std::int64_t os, std::uint8_t* valid, std::int64_t vs, hipStream_t s) {
+  rectify_launch(src, ss, g, out, os, valid, vs, s);
+}
+
+void launch_rectify_u16(const std::uint16_t* src, std::int64_t ss, const RectifyGeom& g, std::uint16_t* out,
+                        std::int64_t os, std::uint8_t* valid, std::int64_t vs, hipStream_t s) {
+  rectify_launch(src, ss, g, out, os, valid, vs, s);
+}
+
+void launch_rectify_f32(const float* src, std::int64_t ss, const RectifyGeom& g, float* out, std::int64_t os,
+                        std::uint8_t* valid, std::int64_t vs, hipStream_t s) {
+  rectify_launch(src, ss, g, out, os, valid, vs, s);
 }
 
 void launch_mirror_u16(const std::uint16_t* in, std::int64_t is, int w, int h, std::uint16_t* out, std::int64_t os,

@@ -20,6 +20,7 @@
 #include <optional>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -1350,6 +1351,84 @@ Server* server_of(Context& c) {
 }
 
 }  // namespace
+// ------------------------------------------------------------------ rectification
+// rectify_image (rectify/rectification.cpp:288-350) for one channel of T.
+template <typename T>
+void rectify(Context& c, const r3dg_image& src, const r3dg_camera& cam, const r3dg_camera& tgt, const r3dg_image& out,
+             const r3dg_image& valid) {
+  if (src.width != cam.width || src.height != cam.height)
+    fail(R3DG_ERROR_INVALID_ARGUMENT, "rectify_image: image is " + std::to_string(src.width) + "x" +
+                                          std::to_string(src.height) + ", camera is " + std::to_string(cam.width) +
+                                          "x" + std::to_string(cam.height));
+  if (cam.model < R3DG_CAMERA_PINHOLE || cam.model > R3DG_CAMERA_FISHEYE)
+    fail(R3DG_ERROR_INVALID_ARGUMENT, "rectify_image: unknown camera model " + std::to_string(cam.model));
+  // Same centre (rectification.cpp:293-295).
+  auto norm3 = [](double a, double b, double cc) { return std::sqrt(a * a + b * b + cc * cc); };
+  const double dc = norm3(cam.center[0] - tgt.center[0], cam.center[1] - tgt.center[1], cam.center[2] - tgt.center[2]);
+  if (dc > 1e-9 * (1.0 + norm3(cam.center[0], cam.center[1], cam.center[2])))
+    fail(R3DG_ERROR_INVALID_ARGUMENT, "rectify_image: the target camera must have the same centre as the source");
+  if (tgt.width <= 0 || tgt.height <= 0) fail(R3DG_ERROR_INVALID_ARGUMENT, "rectify_image: empty target camera");
+  // Anti-aliasing level (rectification.cpp:298-312): the pyramid level that
+  // keeps at least 1/1.5 of the target's resolution.
+  int level = 0;
+  while (cam.fx / tgt.fx > 1.5 * std::ldexp(1.0, level) && (cam.width >> (level + 1)) >= 16 &&
+         (cam.height >> (level + 1)) >= 16)
+    ++level;
+  DImg<T> lv = input_img<T>(c, src, "rectify input");  // empty source: every target pixel is outside
+  for (int l = 0; l < level && !lv.empty(); ++l) {
+    const int dw = (lv.w + 1) / 2, dh = (lv.h + 1) / 2;
+    DImg<T> next = alloc_img<T>(c, dw, dh);
+    if constexpr (std::is_same_v<T, float>) {
+      DBuf<float> tmp(c.arena, std::size_t(dw) * lv.h);
+      launch_pyramid_f32(lv.p, lv.s, lv.w, lv.h, tmp.get(), next.p, next.s, c.stream);
+    } else {
+      DBuf<std::uint32_t> tmp(c.arena, std::size_t(dw) * lv.h);
+      if constexpr (std::is_same_v<T, std::uint16_t>) {
+        launch_pyramid_u16(lv.p, lv.s, lv.w, lv.h, tmp.get(), next.p, next.s, c.stream);
+      } else {
+        launch_pyramid_u8(lv.p, lv.s, lv.w, lv.h, tmp.get(), next.p, next.s, false, c.stream);
+      }
+    }
+    check_launch();
+    lv = std::move(next);
+  }
+  // pyramid_camera (rectification.cpp:152-164) and m = R_src * R_tgt^T.
+  RectifyGeom g{};
+  const double sc = std::ldexp(1.0, -level);
+  g.fx = cam.fx * sc;
+  g.fy = cam.fy * sc;
+  g.cx = cam.cx * sc;
+  g.cy = cam.cy * sc;
+  for (int k = 0; k < 5; ++k) g.params[k] = cam.params[k];
+  g.model = cam.model;
+  g.sw = lv.w;
+  g.sh = lv.h;
+  g.tw = tgt.width;
+  g.th = tgt.height;
+  g.tfx = tgt.fx;
+  g.tfy = tgt.fy;
+  g.tcx = tgt.cx;
+  g.tcy = tgt.cy;
+  const double* R = cam.rotation;
+  const double* Q = tgt.rotation;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) g.m[3 * i + j] = R[3 * i] * Q[3 * j] + R[3 * i + 1] * Q[3 * j + 1] + R[3 * i + 2] * Q[3 * j + 2];
+  DImg<T> o = alloc_img<T>(c, tgt.width, tgt.height);
+  DImg<std::uint8_t> v;
+  if (!view_empty(valid)) v = alloc_img<std::uint8_t>(c, tgt.width, tgt.height);
+  if constexpr (std::is_same_v<T, float>) {
+    launch_rectify_f32(lv.p, lv.s, g, o.p, o.s, v.p, v.s, c.stream);
+  } else if constexpr (std::is_same_v<T, std::uint16_t>) {
+    launch_rectify_u16(lv.p, lv.s, g, o.p, o.s, v.p, v.s, c.stream);
+  } else {
+    launch_rectify_u8(lv.p, lv.s, g, o.p, o.s, v.p, v.s, c.stream);
+  }
+  check_launch();
+  output_img(c, out, o.p, o.w, o.h, o.s, "rectify output");
+  if (!view_empty(valid)) output_img(c, valid, v.p, v.w, v.h, v.s, "rectify valid");
+  sync(c);
+}
+
 }  // namespace r3dg
 
 using namespace r3dg;
@@ -1434,6 +1513,19 @@ int32_t r3dg_cancel(r3dg_ctx* h, int32_t requested) {
 
 int32_t r3dg_automatic_levels(int32_t width, int32_t height, int32_t num_disparities) {
   return automatic_levels(width, height, num_disparities);
+}
+
+int32_t r3dg_rectify(r3dg_ctx* h, int32_t dtype, r3dg_image src, const r3dg_camera* camera,
+                     const r3dg_camera* target, r3dg_image out, r3dg_image valid) {
+  return with_ctx(h, [&](Context& c) {
+    if (camera == nullptr || target == nullptr) fail(R3DG_ERROR_INVALID_ARGUMENT, "rectify_image: null camera");
+    switch (dtype) {
+      case R3DG_U8: rectify<std::uint8_t>(c, src, *camera, *target, out, valid); break;
+      case R3DG_U16: rectify<std::uint16_t>(c, src, *camera, *target, out, valid); break;
+      case R3DG_F32: rectify<float>(c, src, *camera, *target, out, valid); break;
+      default: fail(R3DG_ERROR_INVALID_ARGUMENT, "rectify_image: unknown element type " + std::to_string(dtype));
+    }
+  });
 }
 
 int32_t r3dg_census(r3dg_ctx* h, r3dg_image img, r3dg_image out) {

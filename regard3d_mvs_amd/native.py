@@ -149,6 +149,24 @@ class Matcher:
         self._check(lib().r3dg_census(self._ctx, v, o))
         return out
 
+    # rectify_image<T> (rectify/rectification.hpp:80-82), one channel of uint8,
+    # uint16 or float32: (rectified image, valid mask).
+    def rectify(self, src, camera, target, want_valid=True):
+        dts = {"uint8": (np.uint8, 0), "uint16": (np.uint16, 1), "float32": (np.float32, 2)}
+        key = str(src.dtype).split(".")[-1]
+        if key not in dts:
+            raise TypeError(f"rectify: unsupported element type {src.dtype}")
+        dt, code = dts[key]
+        v, keep = _image(src, dt)
+        shape = (int(target.height), int(target.width))
+        out = _like(src, shape, dt)
+        o, keep2 = _image(out, dt)
+        valid = _like(src, shape, np.uint8) if want_valid else None
+        vv, keep3 = _image(valid, np.uint8)
+        cam, tgt = camera.to_c(), target.to_c()
+        self._check(lib().r3dg_rectify(self._ctx, code, v, ctypes.byref(cam), ctypes.byref(tgt), o, vv))
+        return out, valid
+
     # pyramid_down (pyramid.hpp:20)
     def pyramid_down(self, img):
         dt = np.uint8 if str(img.dtype).endswith("uint8") else np.uint16

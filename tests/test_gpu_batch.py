@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def gpu():
+def gpu(hip_runtime):
     from regard3d_mvs_amd.native import Matcher
     with Matcher(0) as m:
         yield m
