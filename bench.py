@@ -73,6 +73,61 @@ def _same(a, b):
     return a.shape == b.shape and bool(np.array_equal(a.view(np.uint32), b.view(np.uint32)))
 
 
+def bench_rectify(args) -> int:
+    """rectify_image on the 24 MP left image (uint16): a radial_k3 camera
+    turned by a few tenths of a degree, resampled into a 6200x4150 pinhole
+    target, image and mask left in HBM.  HBM-bound gather: the source read
+    once (2 B/px), the image and mask written (3 B per target pixel)."""
+    import torch
+
+    from regard3d_mvs_amd import Camera, CameraModel, synth
+    from regard3d_mvs_amd.native import Matcher
+
+    pair = synth.config_pair(2)
+    h, w = pair.left.shape
+    ax, ay, az = 0.0017, -0.0044, 0.0044  # radians
+    rx = np.array([[1, 0, 0], [0, np.cos(ax), -np.sin(ax)], [0, np.sin(ax), np.cos(ax)]])
+    ry = np.array([[np.cos(ay), 0, np.sin(ay)], [0, 1, 0], [-np.sin(ay), 0, np.cos(ay)]])
+    rz = np.array([[np.cos(az), -np.sin(az), 0], [np.sin(az), np.cos(az), 0], [0, 0, 1]])
+    cam = Camera(w, h, 5200.0, 5200.0, w / 2 - 0.5, h / 2 - 0.5, CameraModel.radial_k3, (-0.06, 0.012, -0.001),
+                 tuple((rz @ ry @ rx).ravel()))
+    tgt = Camera(6200, 4150, 5150.0, 5150.0, 3100.0, 2075.0)
+    src = torch.from_numpy(pair.left.astype(np.int32)).to(torch.uint16).cuda()
+    torch.cuda.synchronize()
+    with Matcher(0) as m:
+        for _ in range(args.warmup):
+            m.rectify(src, cam, tgt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out, valid = m.rectify(src, cam, tgt)
+        torch.cuda.synchronize()
+        secs = (time.perf_counter() - t0) / args.steps
+    tpx = tgt.width * tgt.height
+    alg = 2 * w * h + 3 * tpx
+    cpu = None
+    if args.cpu_rows > 0:
+        from oracle import oracle as O
+        t1 = time.perf_counter()
+        c_out, c_valid = O.rectify(pair.left, cam, tgt)
+        csecs = time.perf_counter() - t1
+        exact = bool(np.array_equal(out.cpu().numpy(), c_out) and np.array_equal(valid.cpu().numpy(), c_valid))
+        cpu = {"value": tpx / csecs / 1e6, "unit": "Mpx/s", "cores": 1, "kind": "port",
+               "sample": f"the whole {tgt.width}x{tgt.height} target in {csecs:.2f} s", "bit_exact": exact}
+    out_j = {"metric": "rectified target Mpx/s (rectify_image, uint16, radial_k3)", "value": tpx / secs / 1e6,
+             "unit": "Mpx/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * secs,
+             "higher_is_better": True, "dtype": "u16", "data": "synthetic",
+             "config": {"workload": f"{w}x{h} uint16 source -> {tgt.width}x{tgt.height} pinhole target, radial_k3",
+                        "inputs": "source resident in HBM, image and mask left in HBM"},
+             "roofline": {"bound": "hbm", "achieved": alg / secs / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                          "frac": alg / secs / HBM_PEAK, "traffic": None, "kernel": "k_rectify<uint16>",
+                          "algorithmic_bytes_per_launch": alg,
+                          "launch_timing": "host wall time per call (one launch, synchronised)"},
+             "cpu_baseline": cpu}
+    print(json.dumps(out_j), flush=True)
+    return 0
+
+
 def bench_batch(args) -> int:
     """The scene of views x 4 partners (1.5-24 MP, spans 64..512) matched as
     plain r3dg_match calls and as one r3dg_match_batch.  value: images already
@@ -229,10 +284,11 @@ def main() -> int:
     ap.add_argument("--config", type=int, default=2, choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-rows", type=int, default=4032, help="rows of the CPU-baseline band (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--workload", default="pair", choices=["pair", "batch", "serve", "copy"],
+    ap.add_argument("--workload", default="pair", choices=["pair", "batch", "serve", "copy", "rectify"],
                     help="pair: one pair per call (default); batch: the 160-pair scene, batched vs plain; "
                          "serve: a Poisson request stream through the serving loop vs plain; copy: one 1 GiB "
-                         "device-to-device copy (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
+                         "device-to-device copy (calibrates the FETCH_SIZE / WRITE_SIZE counters); rectify: "
+                         "rectify_image of the 24 MP image")
     ap.add_argument("--views", type=int, default=40, help="batch: views (4 partners each)")
     ap.add_argument("--requests", type=int, default=300, help="serve: requests per rate")
     ap.add_argument("--rates", default="5,20,50", help="serve: arrival rates (pairs/s)")
@@ -257,6 +313,8 @@ def main() -> int:
         return bench_batch(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "rectify":
+        return bench_rectify(args)
 
     import torch
 

@@ -1413,9 +1413,29 @@ void rectify(Context& c, const r3dg_image& src, const r3dg_camera& cam, const r3
   const double* Q = tgt.rotation;
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) g.m[3 * i + j] = R[3 * i] * Q[3 * j] + R[3 * i + 1] * Q[3 * j + 1] + R[3 * i + 2] * Q[3 * j + 2];
-  DImg<T> o = alloc_img<T>(c, tgt.width, tgt.height);
+  // Device outputs of the target's size are written in place; host ones go
+  // through a device image and the staged download.
+  auto direct = [&](const r3dg_image& im) {
+    return im.memory == R3DG_DEVICE && im.width == tgt.width && im.height == tgt.height && im.stride >= im.width;
+  };
+  DImg<T> o;
+  if (direct(out)) {
+    o.p = static_cast<T*>(out.data);
+    o.w = out.width;
+    o.h = out.height;
+    o.s = out.stride;
+  } else {
+    o = alloc_img<T>(c, tgt.width, tgt.height);
+  }
   DImg<std::uint8_t> v;
-  if (!view_empty(valid)) v = alloc_img<std::uint8_t>(c, tgt.width, tgt.height);
+  if (direct(valid)) {
+    v.p = static_cast<std::uint8_t*>(valid.data);
+    v.w = valid.width;
+    v.h = valid.height;
+    v.s = valid.stride;
+  } else if (!view_empty(valid)) {
+    v = alloc_img<std::uint8_t>(c, tgt.width, tgt.height);
+  }
   if constexpr (std::is_same_v<T, float>) {
     launch_rectify_f32(lv.p, lv.s, g, o.p, o.s, v.p, v.s, c.stream);
   } else if constexpr (std::is_same_v<T, std::uint16_t>) {
@@ -1424,8 +1444,8 @@ void rectify(Context& c, const r3dg_image& src, const r3dg_camera& cam, const r3
     launch_rectify_u8(lv.p, lv.s, g, o.p, o.s, v.p, v.s, c.stream);
   }
   check_launch();
-  output_img(c, out, o.p, o.w, o.h, o.s, "rectify output");
-  if (!view_empty(valid)) output_img(c, valid, v.p, v.w, v.h, v.s, "rectify valid");
+  if (!direct(out)) output_img(c, out, o.p, o.w, o.h, o.s, "rectify output");
+  if (!view_empty(valid) && !direct(valid)) output_img(c, valid, v.p, v.w, v.h, v.s, "rectify valid");
   sync(c);
 }
 
