@@ -114,3 +114,22 @@ def test_rectify_errors(gpu):
         gpu.rectify(src, Camera(13, 10, 1, 1, 0, 0), Camera(12, 10, 1, 1, 0, 0))
     with pytest.raises(R3dgError):
         gpu.rectify(src, Camera(12, 10, 1, 1, 0, 0), Camera(12, 10, 1, 1, 0, 0, center=(0.0, 0.0, 1e-3)))
+
+
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16])
+def test_rectify_unaligned_rows(gpu, dtype):
+    # A device view starting one element into its rows (and rows of an odd
+    # length): the tap loads take the element-wise path.
+    import torch
+    h, w = 70, 101
+    full = source(dtype, h, w + 1, 17)
+    tdt = torch.uint8 if dtype == np.uint8 else torch.uint16
+    view = torch.from_numpy(full.astype(np.int32)).to(tdt).cuda()[:, 1:]
+    src = full[:, 1:].copy()
+    cam = Camera(w, h, 90.0, 90.0, 50.0, 35.0, CameraModel.brown_t2, (-0.05, 0.01, 0.0, 0.001, 0.001),
+                 rot(0.01, 0.0, -0.01).ravel())
+    tgt = Camera(105, 72, 88.0, 88.0, 52.0, 36.0)
+    d_out, d_valid = gpu.rectify(view, cam, tgt)
+    c_out, c_valid = O.rectify(src, cam, tgt)
+    same_bits(d_out.cpu().numpy(), c_out)
+    same_bits(d_valid.cpu().numpy(), c_valid)
