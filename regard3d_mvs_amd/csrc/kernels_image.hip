@@ -344,47 +344,37 @@ __global__ void k_median3x3(const float* __restrict__ in, int w, int h, float* _
     out[std::int64_t(y) * w + x] = c;
     return;
   }
-  float p[9];
+  // The window in registers, outside or non-finite neighbours as +inf: after
+  // a 9-input sorting network the n valid values are the n smallest, in
+  // order, so their median is element n / 2 (and n / 2 - 1 for an even n).
+  // For a full interior window this is the value Paeth's network returns.
+  float v[9];
   int n = 0;
-  bool all = x > 0 && y > 0 && x + 1 < w && y + 1 < h;
-  for (int dy = -1; dy <= 1; ++dy) {
-    for (int dx = -1; dx <= 1; ++dx) {
-      const int qx = x + dx, qy = y + dy;
-      if (qx >= 0 && qy >= 0 && qx < w && qy < h) {
-        const float v = in[std::int64_t(qy) * w + qx];
-        if (finite(v)) {
-          p[n++] = v;
-        } else {
-          all = false;
-        }
-      }
-    }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const int qx = x + k % 3 - 1, qy = y + k / 3 - 1;
+    const bool inside = qx >= 0 && qy >= 0 && qx < w && qy < h;
+    const float a = in[std::int64_t(min(max(qy, 0), h - 1)) * w + min(max(qx, 0), w - 1)];
+    const bool ok = inside && finite(a);
+    v[k] = ok ? a : kInf;
+    n += ok ? 1 : 0;
   }
-  float m;
-  if (all) {
-    auto s2 = [&](int i, int j) {
-      const float a = p[i], b = p[j];
-      p[i] = smin(a, b);
-      p[j] = smax(a, b);
-    };
-    s2(1, 2); s2(4, 5); s2(7, 8); s2(0, 1); s2(3, 4); s2(6, 7);
-    s2(1, 2); s2(4, 5); s2(7, 8); s2(0, 3); s2(5, 8); s2(4, 7);
-    s2(3, 6); s2(1, 4); s2(2, 5); s2(4, 7); s2(4, 2); s2(6, 4);
-    s2(4, 2);
-    m = p[4];
-  } else {
-    for (int i = 1; i < n; ++i) {  // insertion sort of the valid values
-      const float v = p[i];
-      int j = i - 1;
-      while (j >= 0 && v < p[j]) {
-        p[j + 1] = p[j];
-        --j;
-      }
-      p[j + 1] = v;
-    }
-    m = p[n / 2];
-    if (n % 2 == 0) m = 0.5F * (m + p[n / 2 - 1]);
+  auto s2 = [&](int i, int j) {
+    const float a = v[i], b = v[j];
+    v[i] = smin(a, b);
+    v[j] = smax(a, b);
+  };
+  s2(0, 1); s2(3, 4); s2(6, 7); s2(1, 2); s2(4, 5); s2(7, 8); s2(0, 1); s2(3, 4); s2(6, 7);
+  s2(0, 3); s2(3, 6); s2(0, 3); s2(1, 4); s2(4, 7); s2(1, 4); s2(2, 5); s2(5, 8); s2(2, 5);
+  s2(1, 3); s2(5, 7); s2(2, 6); s2(4, 6); s2(2, 4); s2(2, 3); s2(5, 6);
+  const int half = n / 2;
+  float m = v[0], below = v[0];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    m = k == half ? v[k] : m;
+    below = k == half - 1 ? v[k] : below;
   }
+  if (n % 2 == 0) m = 0.5F * (m + below);
   out[std::int64_t(y) * w + x] = m;
 }
 
