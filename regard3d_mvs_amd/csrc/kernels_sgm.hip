@@ -83,6 +83,27 @@ __device__ __forceinline__ void label_range(const JobDesc& J, int X, int Y, int&
 }
 
 // ------------------------------------------------------------------ layout
+// Inclusive scan of v over a block of 256 threads (4 waves): shuffles inside
+// each wave, the wave totals through LDS.  Returns this thread's inclusive
+// prefix and the block total; every thread of the block must call it.
+template <typename U>
+__device__ __forceinline__ U block_scan_incl(U v, U* wsum, U& total) {
+  const int lane = static_cast<int>(threadIdx.x & 63), wave = static_cast<int>(threadIdx.x >> 6);
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const U t = __shfl_up(v, d);
+    if (lane >= d) v += t;
+  }
+  if (lane == 63) wsum[wave] = v;
+  __syncthreads();
+  U off = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) off += k < wave ? wsum[k] : U(0);
+  total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  __syncthreads();  // wsum is reused by the next call
+  return v + off;
+}
+
 // Exclusive scan of the label counts of one tile row (block-wide, in chunks
 // of 256 pixels) into the pixel records: cell offset inside the row, lo, num.
 __global__ __launch_bounds__(256) void k_tile_row_scan(const JobDesc* __restrict__ jobs,
@@ -98,7 +119,7 @@ __global__ __launch_bounds__(256) void k_tile_row_scan(const JobDesc* __restrict
   const JobDesc& J = jobs[T.job];
   const int y = static_cast<int>(r - row_prefix[t]);
   const int Y = T.y0 + y;
-  __shared__ std::uint32_t sc[256];
+  __shared__ std::uint32_t wsum[4];
   __shared__ std::uint32_t smax;
   if (threadIdx.x == 0) smax = 0;
   __syncthreads();
@@ -112,23 +133,16 @@ __global__ __launch_bounds__(256) void k_tile_row_scan(const JobDesc* __restrict
       v = static_cast<std::uint32_t>(num);
       atomicMax(&smax, v);
     }
-    sc[threadIdx.x] = v;
-    __syncthreads();
-    for (int k = 1; k < 256; k <<= 1) {
-      const std::uint32_t add = threadIdx.x >= static_cast<unsigned>(k) ? sc[threadIdx.x - k] : 0U;
-      __syncthreads();
-      sc[threadIdx.x] += add;
-      __syncthreads();
-    }
+    std::uint32_t total;
+    const std::uint32_t incl = block_scan_incl(v, wsum, total);
     if (x < T.w) {
-      const std::uint32_t off = carry + sc[threadIdx.x] - v;
+      const std::uint32_t off = carry + incl - v;
       pix[T.px_base + std::uint64_t(y) * T.w + x] = pix_record(off, lo, num);
       // Row offset of the first pixel of each band.
       if ((x & ((1 << min(T.band_shift, 30)) - 1)) == 0 && (x >> T.band_shift) < T.nbands)
         bstart[T.band_base + std::uint64_t(x >> T.band_shift) * T.h + y] = off;
     }
-    carry += sc[255];
-    __syncthreads();
+    carry += total;
   }
   if (threadIdx.x == 0) {
     row_cells[T.row_base + y] = carry;
@@ -146,34 +160,39 @@ __global__ __launch_bounds__(256) void k_tile_cells(const TileDesc* __restrict__
                                                     std::uint64_t* __restrict__ bcorr,
                                                     std::uint64_t* __restrict__ tile_cells,
                                                     std::uint32_t* __restrict__ tile_max) {
+  constexpr int kItems = 4;  // consecutive band rows per thread and pass
   const TileDesc T = tiles[blockIdx.x];
-  __shared__ std::uint64_t sc[256];
+  __shared__ std::uint64_t wsum[4];
   __shared__ std::uint32_t smax;
   if (threadIdx.x == 0) smax = 0;
   __syncthreads();
   for (int y = threadIdx.x; y < T.h; y += 256) atomicMax(&smax, row_max[T.row_base + y]);
   const int n = T.nbands * T.h;
   std::uint64_t carry = 0;
-  for (int base = 0; base < n; base += 256) {
-    const int e = base + threadIdx.x;
-    std::uint64_t v = 0, b0 = 0;
-    if (e < n) {
-      const int b = e / T.h, y = e - b * T.h;
-      b0 = bstart[T.band_base + e];
-      const std::uint64_t b1 = b + 1 < T.nbands ? bstart[T.band_base + e + T.h] : row_cells[T.row_base + y];
-      v = b1 - b0;
+  for (int base = 0; base < n; base += 256 * kItems) {
+    const int e0 = base + kItems * static_cast<int>(threadIdx.x);
+    std::uint64_t v[kItems], b0[kItems], sum = 0;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      const int e = e0 + i;
+      v[i] = 0;
+      b0[i] = 0;
+      if (e < n) {
+        const int b = e / T.h, y = e - b * T.h;
+        b0[i] = bstart[T.band_base + e];
+        const std::uint64_t b1 = b + 1 < T.nbands ? bstart[T.band_base + e + T.h] : row_cells[T.row_base + y];
+        v[i] = b1 - b0[i];
+      }
+      sum += v[i];
     }
-    sc[threadIdx.x] = v;
-    __syncthreads();
-    for (int k = 1; k < 256; k <<= 1) {
-      const std::uint64_t add = threadIdx.x >= static_cast<unsigned>(k) ? sc[threadIdx.x - k] : 0;
-      __syncthreads();
-      sc[threadIdx.x] += add;
-      __syncthreads();
+    std::uint64_t total;
+    std::uint64_t before = carry + block_scan_incl(sum, wsum, total) - sum;
+#pragma unroll
+    for (int i = 0; i < kItems; ++i) {
+      if (e0 + i < n) bcorr[T.band_base + e0 + i] = before - b0[i];
+      before += v[i];
     }
-    if (e < n) bcorr[T.band_base + e] = carry + sc[threadIdx.x] - v - b0;
-    carry += sc[255];
-    __syncthreads();
+    carry += total;
   }
   if (threadIdx.x == 0) {
     tile_cells[blockIdx.x] = carry;
