@@ -273,6 +273,9 @@ __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, 
   const auto cr = gptr(J.cr) + std::size_t(Y) * J.cr_stride;
   const std::uint64_t* bc = bcorr + T.band_base + y;
   const int pw1 = J.pwidth - 1;
+  const bool mir = J.mirror_census != 0;
+  auto bcol = [&](int X) { return mir ? J.width - 1 - X : X; };  // column of cl
+  auto pcol = [&](int c) { return mir ? pw1 - c : c; };         // column of cr
   // Partner columns a cost of this row can use: X - dmin - label for X in the
   // tile and label in [0, D); every valid column lies in [c0, c1].
   const int c0 = max(0, T.x0 - J.dmin - (J.D - 1)), c1 = min(pw1, T.x0 + T.w - 1 - J.dmin);
@@ -280,7 +283,7 @@ __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, 
   const bool staged = T.w <= kPx && T.nbands <= kBands;
   const std::uint64_t last = rec[T.w - 1];
   if (windowed) {
-    for (int k = threadIdx.x; k <= c1 - c0; k += blockDim.x) swin[k] = cr[c0 + k];
+    for (int k = threadIdx.x; k <= c1 - c0; k += blockDim.x) swin[k] = cr[pcol(c0 + k)];
   }
   if (staged) {
     for (int x = threadIdx.x; x < T.w; x += blockDim.x) {
@@ -288,7 +291,7 @@ __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, 
       const std::uint32_t off = pix_off(pr);
       soff[x] = off;
       slo[x] = static_cast<std::uint16_t>(pix_lo(pr));
-      scl[x] = cl[T.x0 + x];
+      scl[x] = cl[bcol(T.x0 + x)];
       const std::uint32_t e = min((off + static_cast<std::uint32_t>(pix_num(pr))) >> 4, static_cast<std::uint32_t>(kMap));
       for (std::uint32_t k = off >> 4; k < e; ++k) smap[k] = static_cast<std::uint16_t>(x);
     }
@@ -300,7 +303,7 @@ __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, 
   const std::uint32_t invalid = static_cast<std::uint32_t>(J.invalid_cost);
   std::uint8_t* out = C + T.cell_base;
   auto partner = [&](int xr) {
-    return windowed ? swin[min(max(xr, c0), c1) - c0] : cr[min(max(xr, 0), pw1)];
+    return windowed ? swin[min(max(xr, c0), c1) - c0] : cr[pcol(min(max(xr, 0), pw1))];
   };
   // Costs of chunk k of pixel xa (first cell off, first label lo, Census c),
   // stored at the band row offset bco.
@@ -358,7 +361,7 @@ __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, 
         }
       }
       const std::uint64_t pr = rec[xa];
-      chunk(k, xa, pix_off(pr), pix_lo(pr), cl[T.x0 + xa], bc[std::size_t(xa >> T.band_shift) * T.h]);
+      chunk(k, xa, pix_off(pr), pix_lo(pr), cl[bcol(T.x0 + xa)], bc[std::size_t(xa >> T.band_shift) * T.h]);
     }
   }
 }

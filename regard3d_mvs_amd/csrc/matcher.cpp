@@ -276,7 +276,7 @@ struct View {
   DImg<std::uint8_t> mask;
   std::deque<DImg<std::uint16_t>> pyr;  // level l at pyr[l - 1]
   std::deque<DImg<std::uint8_t>> mpyr;
-  std::map<int, DImg<std::uint64_t>> cen, cen_m;
+  std::map<int, DImg<std::uint64_t>> cen;
   std::map<int, DImg<std::uint16_t>> mir;
   std::map<int, DImg<std::uint8_t>> mir_mask;
 
@@ -304,18 +304,12 @@ struct View {
     if (it == mir_mask.end()) it = mir_mask.emplace(l, mirror_u8(c, valid(c, l))).first;
     return it->second;
   }
-  const DImg<std::uint64_t>& mirrored_census(Context& c, int l) {
-    auto it = cen_m.find(l);
-    if (it == cen_m.end()) it = cen_m.emplace(l, census(c, mirrored(c, l))).first;
-    return it->second;
-  }
   // Device bytes held (cache accounting).
   [[nodiscard]] std::uint64_t bytes() const {
     std::uint64_t b = 2 * img.pixels() + mask.pixels();
     for (const auto& x : pyr) b += 2 * x.pixels();
     for (const auto& x : mpyr) b += x.pixels();
     for (const auto& [l, x] : cen) b += 8 * x.pixels();
-    for (const auto& [l, x] : cen_m) b += 8 * x.pixels();
     for (const auto& [l, x] : mir) b += 2 * x.pixels();
     for (const auto& [l, x] : mir_mask) b += x.pixels();
     return b;
@@ -334,6 +328,7 @@ struct Job {
   int D = 0;
   bool confidence = true;
   bool labels = false;
+  bool mirror_census = false;  // cl / cr are the unmirrored images' Census (JobDesc)
 };
 
 DBuf<std::int32_t> pack_rows(Context& c, const std::int32_t* base, const std::int32_t* partner, int h) {
@@ -435,6 +430,7 @@ Aggregation layout_jobs(Context& c, std::vector<Job>& jobs, const r3dg_options& 
     d.cl_stride = J.cl.s;
     d.cr = J.cr.p;
     d.cr_stride = J.cr.s;
+    d.mirror_census = J.mirror_census ? 1 : 0;
     d.rows4 = J.rows4.get();
     d.lo = J.ranges != nullptr ? J.ranges->lo.p : nullptr;
     d.num = J.ranges != nullptr ? J.ranges->num.p : nullptr;
@@ -867,8 +863,10 @@ std::unique_ptr<LevelPrep> prepare_level(Context& c, View& Lv, View& Rv, int l, 
     Job b;
     b.base = &Rv.mirrored(c, l);
     b.partner = &Lv.mirrored(c, l);
-    b.cl = view_of(Rv.mirrored_census(c, l));
-    b.cr = view_of(Lv.mirrored_census(c, l));
+    // The unmirrored images' Census read backwards (JobDesc::mirror_census).
+    b.cl = view_of(Rv.census_at(c, l));
+    b.cr = view_of(Lv.census_at(c, l));
+    b.mirror_census = true;
     b.lrows = P->mrows_base.get();
     b.rrows = P->mrows_partner.get();
     b.rows4 = pack_rows(c, P->mrows_base.get(), P->mrows_partner.get(), h);

@@ -8,12 +8,12 @@
 // and one launch can cover the tiles of many jobs (both directions of a level,
 // or every pair of a batch).
 //
-// HBM layout of the aggregated costs S (uint16): every tile owns a contiguous
-// packed volume at cell_base; inside it, tile row y starts at
-// row_start[row_base + y] and pixel x of that row at the cell offset held in
-// its pixel record pix[px_base + y*w + x] (the TileLabels layout of
+// HBM layout of the aggregated costs S (uint16) and the costs C (uint8): every
+// tile owns a contiguous volume at cell_base, stored band by band (TileDesc
+// below); a pixel's labels follow each other, at the cell offset held in its
+// pixel record pix[px_base + y*w + x] (the TileLabels ranges of
 // stereo_matcher.cpp:172-224).  Label counts are multiples of 16, so every
-// pixel's labels start on a 32-byte boundary.
+// pixel's labels start on a 32-byte boundary of S.
 //
 // Pixel record (uint64): bits 0-31 cell offset inside the tile row, bits 32-47
 // first label `lo`, bits 48-63 label count `num`.
@@ -55,6 +55,12 @@ struct JobDesc {
   std::int32_t p1;
   std::int32_t invalid_cost;
   std::int32_t confidence;   // select writes confidence
+  // 1: cl / cr hold the Census of the unmirrored images of a mirrored job;
+  // base column X reads cl[width - 1 - X], partner column c reads
+  // cr[pwidth - 1 - c].  The Census of a mirrored image is that of the image
+  // read backwards with its bits permuted (neighbour (dy, dx) <-> (dy, -dx)),
+  // and a permutation applied to both sides leaves Hamming distances unchanged.
+  std::int32_t mirror_census;
   std::uint16_t lut[256];    // P2 by 8-bit intensity step (stereo_matcher.cpp:69-79)
 };
 
