@@ -293,6 +293,53 @@ typedef enum r3dg_dtype { R3DG_U8 = 0, R3DG_U16 = 1, R3DG_F32 = 2 } r3dg_dtype;
 int32_t r3dg_rectify(r3dg_ctx* ctx, int32_t dtype, r3dg_image src, const r3dg_camera* camera,
                      const r3dg_camera* target, r3dg_image out, r3dg_image valid);
 
+/* ---- combining the pairs of a view (the step after the matcher) ---------- */
+
+/* Per pair of a view: the PairGeometry of mvs/multi_stereo.cpp:26-33, as the
+ * caller computes it from its RectifiedPair and the grid camera
+ * (multi_stereo.cpp:84-96): grid pixel -> rectified pixel homography
+ * K_r R_r R_g^T K_g^-1 (row-major), f * baseline, disparity offset, the
+ * partner camera's centre; and the pair's maps (float, rectified left image;
+ * confidence may be empty). */
+typedef struct r3dg_pair_geometry {
+  double grid_to_rect[9];
+  double fb;
+  double offset;
+  double partner_center[3];
+  r3dg_image disparity;
+  r3dg_image confidence;
+} r3dg_pair_geometry;
+
+/* The grid camera of the depth map: size, K_g^-1 and R_g^T (row-major) and
+ * its centre (multi_stereo.cpp:84-85, 193-195). */
+typedef struct r3dg_grid {
+  int32_t width;
+  int32_t height;
+  double inv_k[9];
+  double rot_t[9];
+  double center[3];
+} r3dg_grid;
+
+/* MultiStereoOptions (mvs/multi_stereo.hpp:25-30), with cos(min_angle_deg)
+ * precomputed by the caller (multi_stereo.cpp:98). */
+typedef struct r3dg_multi_options {
+  double inlier_px;
+  double max_reprojection_px;
+  double cos_min_angle;
+  int32_t min_pairs;
+  int32_t reserved;
+} r3dg_multi_options;
+
+/* combine_pairs (mvs/multi_stereo.hpp:43-44, .cpp:74-216): per grid pixel,
+ * the hypothesis with the most agreeing pairs, refined by weighted least
+ * squares on inverse depth, kept if it has a majority, min_pairs pairs and a
+ * wide enough triangulation angle.  depth, confidence: float; pair_mask:
+ * uint16 (bit i: pair i used); pixels without a result are 0.  At most 16
+ * pairs (R3DG_ERROR_INVALID_ARGUMENT beyond, as the reference). */
+int32_t r3dg_combine_pairs(r3dg_ctx* ctx, const r3dg_grid* grid, const r3dg_pair_geometry* pairs, int32_t count,
+                           const r3dg_multi_options* options, r3dg_image depth, r3dg_image confidence,
+                           r3dg_image pair_mask);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -33,6 +33,13 @@ oracle::Ranges ranges_in(const std::uint16_t* lo, const std::uint16_t* num, int 
 }
 }  // namespace
 
+template <typename T>
+static oracle::Img<T> copy_strided(const T* p, int w, int h, std::int64_t stride) {
+  oracle::Img<T> m(w, h);
+  for (int y = 0; y < h; ++y) std::memcpy(m.row(y), p + std::int64_t(y) * stride, sizeof(T) * std::size_t(w));
+  return m;
+}
+
 static oracle::Camera camera_of(const r3dg_camera* c) {
   oracle::Camera o;
   o.model = c->model;
@@ -190,6 +197,47 @@ int oracle_rectify_u16(const std::uint16_t* src, int w, int h, const r3dg_camera
 int oracle_rectify_f32(const float* src, int w, int h, const r3dg_camera* cam, const r3dg_camera* target, float* out,
                        std::uint8_t* valid) {
   return rectify_impl(src, w, h, cam, target, out, valid);
+}
+
+float oracle_sample_disparity(const float* d, int w, int h, double u, double v) {
+  float r = 0.0F;
+  guarded([&] { r = oracle::sample_disparity(copy_in(d, w, h), u, v); });
+  return r;
+}
+
+int oracle_combine_pairs(const r3dg_grid* grid, const r3dg_pair_geometry* pairs, int count,
+                         const r3dg_multi_options* options, float* depth, float* confidence, std::uint16_t* mask) {
+  return guarded([&] {
+    oracle::CombineGridIn g;
+    g.width = grid->width;
+    g.height = grid->height;
+    std::memcpy(g.inv_k, grid->inv_k, sizeof(g.inv_k));
+    std::memcpy(g.rot_t, grid->rot_t, sizeof(g.rot_t));
+    std::memcpy(g.center, grid->center, sizeof(g.center));
+    std::vector<oracle::CombinePairIn> ps(static_cast<std::size_t>(count));
+    for (int i = 0; i < count; ++i) {
+      const r3dg_pair_geometry& q = pairs[i];
+      oracle::CombinePairIn& p = ps[static_cast<std::size_t>(i)];
+      std::memcpy(p.grid_to_rect, q.grid_to_rect, sizeof(p.grid_to_rect));
+      p.fb = q.fb;
+      p.offset = q.offset;
+      std::memcpy(p.partner_center, q.partner_center, sizeof(p.partner_center));
+      p.disparity = copy_strided(static_cast<const float*>(q.disparity.data), q.disparity.width, q.disparity.height,
+                                 q.disparity.stride);
+      if (q.confidence.data != nullptr && q.confidence.width > 0 && q.confidence.height > 0)
+        p.confidence = copy_strided(static_cast<const float*>(q.confidence.data), q.confidence.width,
+                                    q.confidence.height, q.confidence.stride);
+    }
+    oracle::CombineOptions o;
+    o.inlier_px = options->inlier_px;
+    o.max_reprojection_px = options->max_reprojection_px;
+    o.cos_min_angle = options->cos_min_angle;
+    o.min_pairs = options->min_pairs;
+    const auto r = oracle::combine_pairs(g, ps, o);
+    std::memcpy(depth, r.depth.px.data(), sizeof(float) * r.depth.px.size());
+    std::memcpy(confidence, r.confidence.px.data(), sizeof(float) * r.confidence.px.size());
+    std::memcpy(mask, r.pair_mask.px.data(), sizeof(std::uint16_t) * r.pair_mask.px.size());
+  });
 }
 
 }  // extern "C"

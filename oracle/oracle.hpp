@@ -18,6 +18,7 @@
 //   median_filter_3x3 KATs of test_stereo.cpp:451-469 + sorted-window median
 //   match_stereo      behaviour thresholds of test_stereo.cpp:191-449
 //   rectify_image     see rectify.cpp
+//   combine_pairs     see combine.cpp
 // Floating-point steps (sub-pixel fit, blending, LR rounding) are restated
 // operation by operation and compiled with -ffp-contract=off.
 #pragma once
@@ -85,6 +86,29 @@ struct Camera {
   double rotation[9] = {};
   double center[3] = {};
 };
+
+// combine_pairs (mvs/multi_stereo.cpp:44-216), per pixel (combine.cpp).
+struct CombinePairIn {
+  double grid_to_rect[9] = {};
+  double fb = 0.0, offset = 0.0;
+  double partner_center[3] = {};
+  ImgF disparity;
+  ImgF confidence;  // may be empty
+};
+struct CombineGridIn {
+  int width = 0, height = 0;
+  double inv_k[9] = {}, rot_t[9] = {}, center[3] = {};
+};
+struct CombineOptions {
+  double inlier_px = 2.0, max_reprojection_px = 1.0, cos_min_angle = 0.0;
+  int min_pairs = 1;
+};
+struct Combined {
+  ImgF depth, confidence;
+  Img<std::uint16_t> pair_mask;
+};
+float sample_disparity(const ImgF& d, double u, double v);
+Combined combine_pairs(const CombineGridIn& g, const std::vector<CombinePairIn>& pairs, const CombineOptions& o);
 
 // rectify_image (rectification.cpp:288-350), one channel (rectify.cpp).
 template <typename T>

@@ -77,6 +77,42 @@ def rectify(src, camera, target):
     return out, valid
 
 
+def sample_disparity(d, u, v):
+    """sample_disparity (mvs/multi_stereo.cpp:44-72)."""
+    d = _c(d, np.float32)
+    h, w = d.shape
+    fn = lib().oracle_sample_disparity
+    fn.restype = ctypes.c_float
+    return float(fn(_ptr(d), w, h, ctypes.c_double(u), ctypes.c_double(v)))
+
+
+def combine_pairs(grid, pairs, options=None):
+    """combine_pairs (mvs/multi_stereo.cpp:74-216): (depth, confidence, pair mask)."""
+    from regard3d_mvs_amd._abi import CImage, CPairGeometry, MultiStereoOptions
+    arr = (CPairGeometry * max(len(pairs), 1))()
+    keep = []
+    for i, p in enumerate(pairs):
+        d = _c(p.disparity, np.float32)
+        keep.append(d)
+        dv = CImage(d.ctypes.data_as(ctypes.c_void_p), d.shape[1], d.shape[0], d.shape[1], 0, 0)
+        cv = CImage(None, 0, 0, 0, 0, 0)
+        if p.confidence is not None:
+            c = _c(p.confidence, np.float32)
+            keep.append(c)
+            cv = CImage(c.ctypes.data_as(ctypes.c_void_p), c.shape[1], c.shape[0], c.shape[1], 0, 0)
+        arr[i] = CPairGeometry((ctypes.c_double * 9)(*[float(v) for v in p.grid_to_rect]), float(p.fb),
+                               float(p.offset), (ctypes.c_double * 3)(*[float(v) for v in p.partner_center]), dv, cv)
+    h, w = int(grid.height), int(grid.width)
+    depth = np.empty((h, w), np.float32)
+    conf = np.empty((h, w), np.float32)
+    mask = np.empty((h, w), np.uint16)
+    g = grid.to_c()
+    o = (options or MultiStereoOptions()).to_c()
+    _check(lib().oracle_combine_pairs(ctypes.byref(g), arr, len(pairs), ctypes.byref(o), _ptr(depth), _ptr(conf),
+                                      _ptr(mask)))
+    return depth, conf, mask
+
+
 def reduce_mask(mask):
     mask = _c(mask, np.uint8)
     h, w = mask.shape

@@ -1,2 +1,3 @@
 """MI355X-native Census-SGM stereo matcher (drop-in for r3d::match_stereo)."""
-from ._abi import Camera, CameraModel, DisparityMap, StereoOptions, SubpixelMethod, align16  # noqa: F401
+from ._abi import (Camera, CameraModel, DisparityMap, GridCamera, MultiStereoOptions, PairGeometry,  # noqa: F401
+                   StereoOptions, SubpixelMethod, align16)

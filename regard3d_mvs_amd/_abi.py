@@ -98,6 +98,62 @@ class Camera:
                        (ctypes.c_double * 9)(*flat(self.rotation, 9)), (ctypes.c_double * 3)(*flat(self.center, 3)))
 
 
+class CPairGeometry(ctypes.Structure):
+    _fields_ = [("grid_to_rect", ctypes.c_double * 9), ("fb", ctypes.c_double), ("offset", ctypes.c_double),
+                ("partner_center", ctypes.c_double * 3), ("disparity", CImage), ("confidence", CImage)]
+
+
+class CGrid(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("inv_k", ctypes.c_double * 9),
+                ("rot_t", ctypes.c_double * 9), ("center", ctypes.c_double * 3)]
+
+
+class CMultiOptions(ctypes.Structure):
+    _fields_ = [("inlier_px", ctypes.c_double), ("max_reprojection_px", ctypes.c_double),
+                ("cos_min_angle", ctypes.c_double), ("min_pairs", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+@dataclasses.dataclass
+class MultiStereoOptions:
+    """r3d::MultiStereoOptions (mvs/multi_stereo.hpp:25-30)."""
+    inlier_px: float = 2.0
+    max_reprojection_px: float = 1.0
+    min_angle_deg: float = 3.0
+    min_pairs: int = 1
+
+    def to_c(self) -> CMultiOptions:
+        import math
+        return CMultiOptions(float(self.inlier_px), float(self.max_reprojection_px),
+                             math.cos(self.min_angle_deg * math.pi / 180.0), int(self.min_pairs), 0)
+
+
+@dataclasses.dataclass
+class GridCamera:
+    """The depth map's camera as combine_pairs uses it: K^-1 and R^T
+    (row-major) and the centre."""
+    width: int
+    height: int
+    inv_k: tuple
+    rot_t: tuple
+    center: tuple
+
+    def to_c(self) -> CGrid:
+        return CGrid(int(self.width), int(self.height), (ctypes.c_double * 9)(*[float(v) for v in self.inv_k]),
+                     (ctypes.c_double * 9)(*[float(v) for v in self.rot_t]),
+                     (ctypes.c_double * 3)(*[float(v) for v in self.center]))
+
+
+@dataclasses.dataclass
+class PairGeometry:
+    """PairGeometry (mvs/multi_stereo.cpp:26-33) with the pair's maps."""
+    grid_to_rect: tuple
+    fb: float
+    offset: float
+    partner_center: tuple
+    disparity: object
+    confidence: object = None
+
+
 STAGES = ("pyramid", "prepare", "ranges", "layout", "aggregate", "select", "blend", "filter")
 
 

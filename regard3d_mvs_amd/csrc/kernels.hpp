@@ -47,6 +47,26 @@ void launch_rectify_u16(const std::uint16_t* src, std::int64_t ss, const Rectify
                         std::int64_t os, std::uint8_t* valid, std::int64_t vs, hipStream_t s);
 void launch_rectify_f32(const float* src, std::int64_t ss, const RectifyGeom& g, float* out, std::int64_t os,
                         std::uint8_t* valid, std::int64_t vs, hipStream_t s);
+// combine_pairs (mvs/multi_stereo.cpp:74-216): one pair of a view (device
+// pointers) and the grid with the options.
+struct CombinePair {
+  double g[9];       // grid pixel -> rectified pixel, row-major
+  double fb, offset;
+  double pc[3];      // partner centre
+  const float* disp;
+  std::int64_t ds;
+  const float* conf;  // may be null
+  std::int64_t cs;
+  int dw, dh, cw, ch;
+};
+struct CombineGrid {
+  double inv_k[9], rot_t[9], center[3];
+  double inlier, max_res, cos_min;
+  int w, h, min_pairs, npairs;
+};
+constexpr int kMaxCombinePairs = 16;  // multi_stereo.cpp:23
+void launch_combine(const CombinePair* pairs, const CombineGrid& g, float* depth, std::int64_t ds, float* conf,
+                    std::int64_t cs, std::uint16_t* mask, std::int64_t ms, hipStream_t s);
 void launch_fill_f32(float* p, std::size_t n, float v, hipStream_t s);
 void launch_fill_u8(std::uint8_t* p, std::size_t n, std::uint8_t v, hipStream_t s);
 

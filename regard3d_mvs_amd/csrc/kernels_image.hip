@@ -494,6 +494,160 @@ __global__ __launch_bounds__(256) void k_rectify(const T* __restrict__ src, std:
   if (valid != nullptr) valid[std::int64_t(y) * vs + x] = inside ? 255 : 0;
 }
 
+// ------------------------------------------------------------------ combining pairs
+// sample_disparity (multi_stereo.cpp:44-72): bilinear when the four
+// neighbours are finite and within one pixel of each other, else the nearest
+// pixel; +inf outside.
+__device__ float sample_disparity(const float* __restrict__ d, std::int64_t s, int w, int h, double u, double v) {
+  if (!(u > -0.5 && v > -0.5 && u < w - 0.5 && v < h - 0.5)) return kInf;
+  const int x0 = min(max(static_cast<int>(floor(u)), 0), w - 1);
+  const int y0 = min(max(static_cast<int>(floor(v)), 0), h - 1);
+  const int x1 = min(x0 + 1, w - 1), y1 = min(y0 + 1, h - 1);
+  const float d00 = d[y0 * s + x0], d10 = d[y0 * s + x1], d01 = d[y1 * s + x0], d11 = d[y1 * s + x1];
+  if (finite(d00) && finite(d10) && finite(d01) && finite(d11)) {
+    const float lo = smin(smin(smin(d00, d10), d01), d11);
+    const float hi = smax(smax(smax(d00, d10), d01), d11);
+    if (hi - lo <= 1.0F) {
+      const double ex = u - x0, ey = v - y0;
+      const float fx = static_cast<float>(ex < 0.0 ? 0.0 : (1.0 < ex ? 1.0 : ex));
+      const float fy = static_cast<float>(ey < 0.0 ? 0.0 : (1.0 < ey ? 1.0 : ey));
+      const float top = d00 + fx * (d10 - d00);
+      const float bottom = d01 + fx * (d11 - d01);
+      return top + fy * (bottom - top);
+    }
+  }
+  const int xn = min(max(static_cast<int>(lround(u)), 0), w - 1);
+  const int yn = min(max(static_cast<int>(lround(v)), 0), h - 1);
+  return d[yn * s + xn];
+}
+
+// Unit vector a / |a| (Eigen's normalized(): unchanged when |a| = 0).
+__device__ __forceinline__ void unit3(double* a) {
+  const double n2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+  if (n2 > 0.0) {
+    const double n = sqrt(n2);
+    a[0] = a[0] / n;
+    a[1] = a[1] / n;
+    a[2] = a[2] / n;
+  }
+}
+
+// combine_pairs per grid pixel (multi_stereo.cpp:103-213), double precision
+// in the reference's order of operations.
+__global__ __launch_bounds__(256) void k_combine(const CombinePair* __restrict__ pairs, CombineGrid g,
+                                                 float* __restrict__ depth, std::int64_t ds,
+                                                 float* __restrict__ conf_out, std::int64_t cs,
+                                                 std::uint16_t* __restrict__ mask_out, std::int64_t ms) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, yy = blockIdx.y;
+  if (x >= g.w) return;
+  double cw[kMaxCombinePairs], ck[kMaxCombinePairs];
+  float cc[kMaxCombinePairs];
+  int cp[kMaxCombinePairs], used[kMaxCombinePairs];
+  int n = 0;
+  for (int i = 0; i < g.npairs; ++i) {
+    const CombinePair& P = pairs[i];
+    const double rx = P.g[0] * x + P.g[1] * yy + P.g[2] * 1.0;
+    const double ry = P.g[3] * x + P.g[4] * yy + P.g[5] * 1.0;
+    const double c = P.g[6] * x + P.g[7] * yy + P.g[8] * 1.0;
+    if (!(c > 1e-9)) continue;
+    const double u = rx / c, v = ry / c;
+    const float d = sample_disparity(P.disp, P.ds, P.dw, P.dh, u, v);
+    if (!finite(d) || d <= P.offset) continue;
+    ck[n] = P.fb / c;
+    cw[n] = (d - P.offset) / ck[n];
+    cp[n] = i;
+    float cf = 0.0F;
+    if (P.conf != nullptr) {
+      const int xn = min(max(static_cast<int>(lround(u)), 0), P.cw - 1);
+      const int yn = min(max(static_cast<int>(lround(v)), 0), P.ch - 1);
+      cf = P.conf[yn * P.cs + xn];
+    }
+    cc[n] = cf;
+    ++n;
+  }
+  float dep = 0.0F, cnf = 0.0F;
+  std::uint16_t mask = 0;
+  if (n >= g.min_pairs) {
+    // The hypothesis with the most agreeing pairs (ties: higher summed confidence).
+    int best = -1, best_support = 0;
+    float best_conf = -1.0F;
+    for (int i = 0; i < n; ++i) {
+      int support = 0;
+      float sc = 0.0F;
+      for (int j = 0; j < n; ++j) {
+        if (fabs(ck[j] * (cw[j] - cw[i])) <= g.inlier) {
+          ++support;
+          sc += cc[j];
+        }
+      }
+      if (support > best_support || (support == best_support && sc > best_conf)) {
+        best = i;
+        best_support = support;
+        best_conf = sc;
+      }
+    }
+    int m = 0;
+    for (int j = 0; j < n; ++j)
+      if (fabs(ck[j] * (cw[j] - cw[best])) <= g.inlier) used[m++] = j;
+    // Weighted least squares on w, dropping the worst pair while a residual is too large.
+    double w_opt = 0.0;
+    while (m > 0) {
+      double sw = 0.0, swx = 0.0;
+      for (int t = 0; t < m; ++t) {
+        const int q = used[t];
+        sw += ck[q] * ck[q];
+        swx += ck[q] * ck[q] * cw[q];
+      }
+      w_opt = swx / sw;
+      int worst = -1;
+      double worst_res = g.max_res;
+      for (int t = 0; t < m; ++t) {
+        const int q = used[t];
+        const double res = fabs(ck[q] * (cw[q] - w_opt));
+        if (res > worst_res) {
+          worst_res = res;
+          worst = t;
+        }
+      }
+      if (worst < 0) break;
+      used[worst] = used[--m];
+    }
+    const bool majority = m > n - m || (m >= 2 && m >= n - m);
+    if (m >= g.min_pairs && majority && w_opt > 0.0) {
+      const double z = 1.0 / w_opt;
+      const double px[3] = {static_cast<double>(x), static_cast<double>(yy), 1.0};
+      double t[3], xw[3], tb[3];
+      for (int i = 0; i < 3; ++i) t[i] = g.inv_k[3 * i] * px[0] + g.inv_k[3 * i + 1] * px[1] + g.inv_k[3 * i + 2] * px[2];
+      for (int i = 0; i < 3; ++i) {
+        const double ray = g.rot_t[3 * i] * t[0] + g.rot_t[3 * i + 1] * t[1] + g.rot_t[3 * i + 2] * t[2];
+        xw[i] = g.center[i] + z * ray;
+        tb[i] = g.center[i] - xw[i];
+      }
+      unit3(tb);
+      bool wide = false;
+      float sc = 0.0F;
+      for (int k = 0; k < m; ++k) {
+        const int q = used[k];
+        mask = static_cast<std::uint16_t>(mask | (1U << cp[q]));
+        const CombinePair& P = pairs[cp[q]];
+        double tp[3] = {P.pc[0] - xw[0], P.pc[1] - xw[1], P.pc[2] - xw[2]};
+        unit3(tp);
+        wide = wide || tb[0] * tp[0] + tb[1] * tp[1] + tb[2] * tp[2] <= g.cos_min;
+        sc += cc[q];
+      }
+      if (wide) {
+        dep = static_cast<float>(z);
+        cnf = sc / static_cast<float>(m);
+      } else {
+        mask = 0;
+      }
+    }
+  }
+  depth[yy * ds + x] = dep;
+  conf_out[yy * cs + x] = cnf;
+  mask_out[yy * ms + x] = mask;
+}
+
 // ------------------------------------------------------------------ label ranges
 // ranges_from_coarse (stereo_matcher.cpp:1124-1267).
 __global__ void k_rng_h(const float* __restrict__ coarse, int wc, int hc, int r, float* __restrict__ hmin,
@@ -661,6 +815,12 @@ void launch_rectify_u16(const std::uint16_t* src, std::int64_t ss, const Rectify
 void launch_rectify_f32(const float* src, std::int64_t ss, const RectifyGeom& g, float* out, std::int64_t os,
                         std::uint8_t* valid, std::int64_t vs, hipStream_t s) {
   rectify_launch(src, ss, g, out, os, valid, vs, s);
+}
+
+void launch_combine(const CombinePair* pairs, const CombineGrid& g, float* depth, std::int64_t ds, float* conf,
+                    std::int64_t cs, std::uint16_t* mask, std::int64_t ms, hipStream_t s) {
+  if (g.w <= 0 || g.h <= 0) return;
+  k_combine<<<dim3((g.w + 255) / 256, g.h), 256, 0, s>>>(pairs, g, depth, ds, conf, cs, mask, ms);
 }
 
 void launch_mirror_u16(const std::uint16_t* in, std::int64_t is, int w, int h, std::uint16_t* out, std::int64_t os,

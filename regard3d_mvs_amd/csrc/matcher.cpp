@@ -1533,6 +1533,70 @@ int32_t r3dg_automatic_levels(int32_t width, int32_t height, int32_t num_dispari
   return automatic_levels(width, height, num_disparities);
 }
 
+int32_t r3dg_combine_pairs(r3dg_ctx* h, const r3dg_grid* grid, const r3dg_pair_geometry* pairs, int32_t count,
+                           const r3dg_multi_options* options, r3dg_image depth, r3dg_image confidence,
+                           r3dg_image pair_mask) {
+  return with_ctx(h, [&](Context& c) {
+    if (grid == nullptr || options == nullptr) fail(R3DG_ERROR_INVALID_ARGUMENT, "combine_pairs: null grid or options");
+    if (count < 0 || count > kMaxCombinePairs)
+      fail(R3DG_ERROR_INVALID_ARGUMENT, "combine_pairs: at most " + std::to_string(kMaxCombinePairs) + " pairs per view (" +
+                                            std::to_string(count) + " given)");
+    if (count > 0 && pairs == nullptr) fail(R3DG_ERROR_INVALID_ARGUMENT, "combine_pairs: no pairs");
+    const int w = grid->width, hh = grid->height;
+    if (w < 0 || hh < 0) fail(R3DG_ERROR_INVALID_ARGUMENT, "combine_pairs: negative grid size");
+    // The pairs' maps on the device (host maps are uploaded once).
+    std::vector<DImg<float>> maps;
+    maps.reserve(2 * std::size_t(count));
+    std::vector<CombinePair> cp(static_cast<std::size_t>(count));
+    for (int i = 0; i < count; ++i) {
+      const r3dg_pair_geometry& q = pairs[i];
+      if (view_empty(q.disparity)) fail(R3DG_ERROR_INVALID_ARGUMENT, "combine_pairs: empty disparity map");
+      maps.push_back(input_img<float>(c, q.disparity, "combine disparity"));
+      const DImg<float>& d = maps.back();
+      CombinePair& P = cp[std::size_t(i)];
+      for (int k = 0; k < 9; ++k) P.g[k] = q.grid_to_rect[k];
+      P.fb = q.fb;
+      P.offset = q.offset;
+      for (int k = 0; k < 3; ++k) P.pc[k] = q.partner_center[k];
+      P.disp = d.p;
+      P.ds = d.s;
+      P.dw = d.w;
+      P.dh = d.h;
+      if (!view_empty(q.confidence)) {
+        maps.push_back(input_img<float>(c, q.confidence, "combine confidence"));
+        const DImg<float>& cf = maps.back();
+        P.conf = cf.p;
+        P.cs = cf.s;
+        P.cw = cf.w;
+        P.ch = cf.h;
+      }
+    }
+    DBuf<CombinePair> dpairs;
+    upload(c, dpairs, cp);
+    CombineGrid g{};
+    for (int k = 0; k < 9; ++k) {
+      g.inv_k[k] = grid->inv_k[k];
+      g.rot_t[k] = grid->rot_t[k];
+    }
+    for (int k = 0; k < 3; ++k) g.center[k] = grid->center[k];
+    g.inlier = options->inlier_px;
+    g.max_res = options->max_reprojection_px;
+    g.cos_min = options->cos_min_angle;
+    g.min_pairs = std::max(1, options->min_pairs);  // multi_stereo.cpp:99
+    g.npairs = count;
+    g.w = w;
+    g.h = hh;
+    DImg<float> dep = alloc_img<float>(c, w, hh), cnf = alloc_img<float>(c, w, hh);
+    DImg<std::uint16_t> msk = alloc_img<std::uint16_t>(c, w, hh);
+    launch_combine(dpairs.get(), g, dep.p, dep.s, cnf.p, cnf.s, msk.p, msk.s, c.stream);
+    check_launch();
+    output_img(c, depth, dep.p, dep.w, dep.h, dep.s, "combine depth");
+    output_img(c, confidence, cnf.p, cnf.w, cnf.h, cnf.s, "combine confidence");
+    output_img(c, pair_mask, msk.p, msk.w, msk.h, msk.s, "combine pair mask");
+    sync(c);
+  });
+}
+
 int32_t r3dg_rectify(r3dg_ctx* h, int32_t dtype, r3dg_image src, const r3dg_camera* camera,
                      const r3dg_camera* target, r3dg_image out, r3dg_image valid) {
   return with_ctx(h, [&](Context& c) {

@@ -167,6 +167,30 @@ class Matcher:
         self._check(lib().r3dg_rectify(self._ctx, code, v, ctypes.byref(cam), ctypes.byref(tgt), o, vv))
         return out, valid
 
+    # combine_pairs (mvs/multi_stereo.hpp:43-44): (depth, confidence, pair mask)
+    # on the grid; `pairs` are PairGeometry records with their maps.
+    def combine_pairs(self, grid, pairs, options=None):
+        from ._abi import CPairGeometry, MultiStereoOptions
+        arr = (CPairGeometry * max(len(pairs), 1))()
+        keep = []
+        for i, p in enumerate(pairs):
+            dv, k1 = _image(p.disparity, np.float32)
+            cv, k2 = _image(p.confidence, np.float32)
+            keep += [k1, k2]
+            arr[i] = CPairGeometry((ctypes.c_double * 9)(*[float(v) for v in p.grid_to_rect]), float(p.fb),
+                                   float(p.offset), (ctypes.c_double * 3)(*[float(v) for v in p.partner_center]),
+                                   dv, cv)
+        shape = (int(grid.height), int(grid.width))
+        ref = pairs[0].disparity if pairs else None
+        depth, conf, mask = _like(ref, shape, np.float32), _like(ref, shape, np.float32), _like(ref, shape, np.uint16)
+        d, k3 = _image(depth, np.float32)
+        c, k4 = _image(conf, np.float32)
+        m, k5 = _image(mask, np.uint16)
+        g = grid.to_c()
+        o = (options or MultiStereoOptions()).to_c()
+        self._check(lib().r3dg_combine_pairs(self._ctx, ctypes.byref(g), arr, len(pairs), ctypes.byref(o), d, c, m))
+        return depth, conf, mask
+
     # pyramid_down (pyramid.hpp:20)
     def pyramid_down(self, img):
         dt = np.uint8 if str(img.dtype).endswith("uint8") else np.uint16
