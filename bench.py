@@ -128,6 +128,60 @@ def bench_rectify(args) -> int:
     return 0
 
 
+def bench_combine(args) -> int:
+    """combine_pairs on a 6048x4536 grid (the reference rig of test_mvs.cpp
+    scaled by 18.9) from four rectified pairs of the same size, maps in HBM,
+    one pair 6 pixels off.  Per grid pixel: 4 bilinear samples of 4 bytes and
+    a 4-byte confidence per pair read, 10 bytes written."""
+    import torch
+
+    from regard3d_mvs_amd import PairGeometry
+    from regard3d_mvs_amd.multiview import Rig
+    from regard3d_mvs_amd.native import Matcher
+
+    offsets = ((0.8, 0.05, 0.1), (-0.7, 0.1, 0.0), (0.1, 0.75, -0.1), (-0.2, -0.7, 0.05))
+    rig = Rig(offsets, scale=18.9)
+    host = [rig.pair(c, shift=6.0 if i == 2 else 0.0) for i, c in enumerate(rig.partners)]
+    dev = [PairGeometry(p.grid_to_rect, p.fb, p.offset, p.partner_center, torch.from_numpy(p.disparity).cuda(),
+                        torch.from_numpy(p.confidence).cuda()) for p in host]
+    grid = rig.grid()
+    torch.cuda.synchronize()
+    with Matcher(0) as m:
+        for _ in range(args.warmup):
+            m.combine_pairs(grid, dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            depth, conf, mask = m.combine_pairs(grid, dev)
+        torch.cuda.synchronize()
+        secs = (time.perf_counter() - t0) / args.steps
+    px = grid.width * grid.height
+    alg = px * (len(host) * 8 + 10)  # disparity (4) + confidence (4) per pair read, 10 written
+    cpu = None
+    if args.cpu_rows > 0:
+        from oracle import oracle as O
+        t1 = time.perf_counter()
+        c = O.combine_pairs(grid, host)
+        csecs = time.perf_counter() - t1
+        exact = all(np.array_equal(a.cpu().numpy().view(np.uint8), b.view(np.uint8)) for a, b in zip((depth, conf, mask), c))
+        cpu = {"value": px / csecs / 1e6, "unit": "Mpx/s", "cores": 1, "kind": "port",
+               "sample": f"the whole {grid.width}x{grid.height} grid in {csecs:.2f} s", "bit_exact": bool(exact)}
+    valid = float((depth > 0).float().mean().item())
+    out_j = {"metric": "combined grid Mpx/s (combine_pairs, 4 pairs)", "value": px / secs / 1e6, "unit": "Mpx/s",
+             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * secs,
+             "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+             "config": {"workload": f"{grid.width}x{grid.height} grid, 4 rectified pairs of a slanted plane, "
+                                    "one 6 px off", "valid": valid,
+                        "inputs": "maps resident in HBM, depth / confidence / mask left in HBM"},
+             "roofline": {"bound": "hbm", "achieved": alg / secs / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                          "frac": alg / secs / HBM_PEAK, "traffic": None, "kernel": "k_combine",
+                          "algorithmic_bytes_per_launch": alg,
+                          "launch_timing": "host wall time per call (one launch, synchronised)"},
+             "cpu_baseline": cpu}
+    print(json.dumps(out_j), flush=True)
+    return 0
+
+
 def bench_batch(args) -> int:
     """The scene of views x 4 partners (1.5-24 MP, spans 64..512) matched as
     plain r3dg_match calls and as one r3dg_match_batch.  value: images already
@@ -284,11 +338,11 @@ def main() -> int:
     ap.add_argument("--config", type=int, default=2, choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-rows", type=int, default=4032, help="rows of the CPU-baseline band (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--workload", default="pair", choices=["pair", "batch", "serve", "copy", "rectify"],
+    ap.add_argument("--workload", default="pair", choices=["pair", "batch", "serve", "copy", "rectify", "combine"],
                     help="pair: one pair per call (default); batch: the 160-pair scene, batched vs plain; "
                          "serve: a Poisson request stream through the serving loop vs plain; copy: one 1 GiB "
                          "device-to-device copy (calibrates the FETCH_SIZE / WRITE_SIZE counters); rectify: "
-                         "rectify_image of the 24 MP image")
+                         "rectify_image of the 24 MP image; combine: combine_pairs of 4 pairs on a 24 MP grid")
     ap.add_argument("--views", type=int, default=40, help="batch: views (4 partners each)")
     ap.add_argument("--requests", type=int, default=300, help="serve: requests per rate")
     ap.add_argument("--rates", default="5,20,50", help="serve: arrival rates (pairs/s)")
@@ -315,6 +369,8 @@ def main() -> int:
         return bench_serve(args)
     if args.workload == "rectify":
         return bench_rectify(args)
+    if args.workload == "combine":
+        return bench_combine(args)
 
     import torch
 

@@ -1586,13 +1586,30 @@ int32_t r3dg_combine_pairs(r3dg_ctx* h, const r3dg_grid* grid, const r3dg_pair_g
     g.npairs = count;
     g.w = w;
     g.h = hh;
-    DImg<float> dep = alloc_img<float>(c, w, hh), cnf = alloc_img<float>(c, w, hh);
-    DImg<std::uint16_t> msk = alloc_img<std::uint16_t>(c, w, hh);
+    // Device outputs of the grid's size are written in place.
+    auto direct = [&](const r3dg_image& im) {
+      return im.memory == R3DG_DEVICE && im.width == w && im.height == hh && im.stride >= im.width;
+    };
+    auto target = [&](const r3dg_image& im, auto* type_tag) {
+      using T = std::remove_pointer_t<decltype(type_tag)>;
+      DImg<T> d;
+      if (direct(im)) {
+        d.p = static_cast<T*>(im.data);
+        d.w = w;
+        d.h = hh;
+        d.s = im.stride;
+      } else {
+        d = alloc_img<T>(c, w, hh);
+      }
+      return d;
+    };
+    DImg<float> dep = target(depth, static_cast<float*>(nullptr)), cnf = target(confidence, static_cast<float*>(nullptr));
+    DImg<std::uint16_t> msk = target(pair_mask, static_cast<std::uint16_t*>(nullptr));
     launch_combine(dpairs.get(), g, dep.p, dep.s, cnf.p, cnf.s, msk.p, msk.s, c.stream);
     check_launch();
-    output_img(c, depth, dep.p, dep.w, dep.h, dep.s, "combine depth");
-    output_img(c, confidence, cnf.p, cnf.w, cnf.h, cnf.s, "combine confidence");
-    output_img(c, pair_mask, msk.p, msk.w, msk.h, msk.s, "combine pair mask");
+    if (!direct(depth)) output_img(c, depth, dep.p, dep.w, dep.h, dep.s, "combine depth");
+    if (!direct(confidence)) output_img(c, confidence, cnf.p, cnf.w, cnf.h, cnf.s, "combine confidence");
+    if (!direct(pair_mask)) output_img(c, pair_mask, msk.p, msk.w, msk.h, msk.s, "combine pair mask");
     sync(c);
   });
 }

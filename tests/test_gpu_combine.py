@@ -5,7 +5,7 @@ disagreeing pairs and up to the 16-pair limit."""
 import numpy as np
 import pytest
 
-from multiview_scene import Rig
+from regard3d_mvs_amd.multiview import Rig
 from oracle import oracle as O
 from regard3d_mvs_amd import MultiStereoOptions, PairGeometry
 

@@ -1,12 +1,12 @@
 """Pins the CPU oracle's combine_pairs (mvs/multi_stereo.cpp:44-216) with the
 reference's own tests restated on a rig built in numpy
-(tests/multiview_scene.py): sample_disparity KATs (test_mvs.cpp:217-227), a
+(regard3d_mvs_amd/multiview.py): sample_disparity KATs (test_mvs.cpp:217-227), a
 slanted plane triangulated by three pairs with the wrong one outvoted
 (:130-172), and the rejection of ties, single pairs under min_pairs = 2 and
 narrow triangulation angles (:174-215)."""
 import numpy as np
 
-from multiview_scene import Rig
+from regard3d_mvs_amd.multiview import Rig
 from oracle import oracle as O
 from regard3d_mvs_amd import MultiStereoOptions
 
