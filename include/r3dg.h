@@ -340,6 +340,47 @@ int32_t r3dg_combine_pairs(r3dg_ctx* ctx, const r3dg_grid* grid, const r3dg_pair
                            const r3dg_multi_options* options, r3dg_image depth, r3dg_image confidence,
                            r3dg_image pair_mask);
 
+/* ---- file formats (host only; no context, errors via r3dg_global_error) ---- */
+
+/* read_pfm / write_pfm (io/pfm.hpp, io/pfm.cpp:30-102): "Pf" (1 channel) or
+ * "PF" (3, interleaved), rows stored bottom to top, little-endian when the
+ * scale is negative (written as -1).  r3dg_pfm_header reads the size; out
+ * receives height rows of width * channels floats, `stride` floats apart. */
+int32_t r3dg_pfm_header(const char* path, int32_t* width, int32_t* height, int32_t* channels);
+int32_t r3dg_read_pfm(const char* path, float* out, int64_t stride);
+int32_t r3dg_write_pfm(const char* path, const float* data, int32_t width, int32_t height, int32_t channels,
+                       int64_t stride);
+
+/* The .r3dd depth map file of write_depth_map / read_depth_map
+ * (mvs/depth_map.hpp:59-62, depth_map.cpp:94-221), version 2: view, the
+ * grid camera (intrinsics, row-major rotation, centre), up to 16 partner
+ * views, then depth (float), confidence (float), pair mask (uint16), support
+ * (uint8) and colour (uint8 RGB) of width x height, rows contiguous. */
+typedef struct r3dg_depth_map {
+  uint32_t view;
+  int32_t camera_width;
+  int32_t camera_height;
+  uint32_t num_partners;
+  double fx, fy, cx, cy;
+  double rotation[9];
+  double center[3];
+  uint32_t partners[16];
+  int32_t width;
+  int32_t height;
+  float* depth;
+  float* confidence;
+  uint16_t* pair_mask;
+  uint8_t* support;
+  uint8_t* color;
+} r3dg_depth_map;
+
+/* Reads everything but the images (the caller then sizes the buffers). */
+int32_t r3dg_read_depth_map_header(const char* path, r3dg_depth_map* map);
+/* Reads the whole file into `map`, whose image pointers must hold
+ * width * height elements (colour: 3 per pixel). */
+int32_t r3dg_read_depth_map(const char* path, r3dg_depth_map* map);
+int32_t r3dg_write_depth_map(const char* path, const r3dg_depth_map* map);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

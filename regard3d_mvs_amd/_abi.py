@@ -154,6 +154,37 @@ class PairGeometry:
     confidence: object = None
 
 
+class CDepthMap(ctypes.Structure):
+    _fields_ = [("view", ctypes.c_uint32), ("camera_width", ctypes.c_int32), ("camera_height", ctypes.c_int32),
+                ("num_partners", ctypes.c_uint32), ("fx", ctypes.c_double), ("fy", ctypes.c_double),
+                ("cx", ctypes.c_double), ("cy", ctypes.c_double), ("rotation", ctypes.c_double * 9),
+                ("center", ctypes.c_double * 3), ("partners", ctypes.c_uint32 * 16), ("width", ctypes.c_int32),
+                ("height", ctypes.c_int32), ("depth", ctypes.c_void_p), ("confidence", ctypes.c_void_p),
+                ("pair_mask", ctypes.c_void_p), ("support", ctypes.c_void_p), ("color", ctypes.c_void_p)]
+
+
+@dataclasses.dataclass
+class DepthMapFile:
+    """The contents of a .r3dd file (mvs/depth_map.hpp:16-45): view, grid camera
+    (intrinsics of camera_width x camera_height, row-major rotation, centre),
+    partner views and the five images of width x height."""
+    view: int
+    camera_width: int
+    camera_height: int
+    fx: float
+    fy: float
+    cx: float
+    cy: float
+    rotation: tuple
+    center: tuple
+    partners: tuple
+    depth: object       # float32 (h, w)
+    confidence: object  # float32 (h, w)
+    pair_mask: object   # uint16 (h, w)
+    support: object     # uint8 (h, w)
+    color: object       # uint8 (h, w, 3)
+
+
 STAGES = ("pyramid", "prepare", "ranges", "layout", "aggregate", "select", "blend", "filter")
 
 

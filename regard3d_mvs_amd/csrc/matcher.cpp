@@ -15,6 +15,10 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <stdexcept>
+#include <cctype>
+#include <iterator>
+#include <fstream>
 #include <map>
 #include <memory>
 #include <optional>
@@ -1447,6 +1451,110 @@ void rectify(Context& c, const r3dg_image& src, const r3dg_camera& cam, const r3
   sync(c);
 }
 
+// ---- file formats (io/pfm.cpp, mvs/depth_map.cpp) -----------------------
+std::vector<std::uint8_t> read_bytes(const char* path) {
+  if (path == nullptr) fail(R3DG_ERROR_INVALID_ARGUMENT, "null path");
+  std::ifstream f(path, std::ios::binary);
+  if (!f) fail(R3DG_ERROR_NOT_FOUND, std::string("cannot open ") + path);
+  return std::vector<std::uint8_t>((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+}
+
+void write_bytes(const char* path, const std::vector<std::uint8_t>& data) {
+  if (path == nullptr) fail(R3DG_ERROR_INVALID_ARGUMENT, "null path");
+  std::ofstream f(path, std::ios::binary);
+  if (!f) fail(R3DG_ERROR_INVALID_ARGUMENT, std::string("cannot write ") + path);
+  f.write(reinterpret_cast<const char*>(data.data()), static_cast<std::streamsize>(data.size()));
+  if (!f) fail(R3DG_ERROR_INTERNAL, std::string("error writing ") + path);
+}
+
+// PFM header (pfm.cpp:30-62): magic, width, height, scale; the data start
+// one whitespace character after the scale.
+struct PfmHeader {
+  int width = 0, height = 0, channels = 0;
+  bool little = true;
+  std::size_t data = 0;
+};
+
+PfmHeader pfm_header(const std::vector<std::uint8_t>& d, const char* path) {
+  std::size_t pos = 0;
+  auto token = [&]() {
+    while (pos < d.size() && std::isspace(d[pos]) != 0) ++pos;
+    std::string t;
+    while (pos < d.size() && std::isspace(d[pos]) == 0) t += static_cast<char>(d[pos++]);
+    return t;
+  };
+  PfmHeader h;
+  const std::string magic = token();
+  if (magic == "Pf") {
+    h.channels = 1;
+  } else if (magic == "PF") {
+    h.channels = 3;
+  } else {
+    fail(R3DG_ERROR_INVALID_ARGUMENT, std::string(path) + ": not a PFM file");
+  }
+  double scale = 0.0;
+  try {
+    h.width = std::stoi(token());
+    h.height = std::stoi(token());
+    scale = std::stod(token());
+  } catch (const std::exception&) {
+    fail(R3DG_ERROR_INVALID_ARGUMENT, std::string(path) + ": invalid PFM header");
+  }
+  ++pos;
+  if (h.width <= 0 || h.height <= 0 || scale == 0.0) fail(R3DG_ERROR_INVALID_ARGUMENT, std::string(path) + ": invalid PFM header");
+  h.little = scale < 0.0;
+  h.data = pos;
+  const std::size_t bytes = std::size_t(h.width) * std::size_t(h.channels) * std::size_t(h.height) * 4;
+  if (d.size() < pos + bytes) fail(R3DG_ERROR_INVALID_ARGUMENT, std::string(path) + ": truncated PFM data");
+  return h;
+}
+
+// .r3dd (depth_map.cpp:94-221), native little-endian PODs.
+constexpr char kDepthMagic[4] = {'R', '3', 'D', 'D'};
+constexpr std::uint32_t kDepthVersion = 2;
+
+struct ByteReader {
+  const std::vector<std::uint8_t>& d;
+  std::size_t pos = 0;
+  template <typename T>
+  T pod() {
+    if (pos + sizeof(T) > d.size()) throw std::out_of_range("truncated");
+    T v;
+    std::memcpy(&v, d.data() + pos, sizeof(T));
+    pos += sizeof(T);
+    return v;
+  }
+  void bytes(void* out, std::size_t n) {
+    if (pos + n > d.size()) throw std::out_of_range("truncated");
+    std::memcpy(out, d.data() + pos, n);
+    pos += n;
+  }
+};
+
+void depth_header(ByteReader& r, r3dg_depth_map& m, const char* path) {
+  char magic[4];
+  r.bytes(magic, 4);
+  const auto version = r.pod<std::uint32_t>();
+  if (std::memcmp(magic, kDepthMagic, 4) != 0 || version != kDepthVersion)
+    fail(R3DG_ERROR_INVALID_ARGUMENT, std::string(path) + " is not an r3d depth map (version 2)");
+  m.view = r.pod<std::uint32_t>();
+  m.camera_width = r.pod<std::int32_t>();
+  m.camera_height = r.pod<std::int32_t>();
+  m.fx = r.pod<double>();
+  m.fy = r.pod<double>();
+  m.cx = r.pod<double>();
+  m.cy = r.pod<double>();
+  for (double& v : m.rotation) v = r.pod<double>();
+  for (double& v : m.center) v = r.pod<double>();
+  m.num_partners = r.pod<std::uint32_t>();
+  if (m.num_partners > 16) fail(R3DG_ERROR_INVALID_ARGUMENT, std::string("corrupt depth map ") + path);
+  for (std::uint32_t i = 0; i < m.num_partners; ++i) m.partners[i] = r.pod<std::uint32_t>();
+  m.width = r.pod<std::int32_t>();
+  m.height = r.pod<std::int32_t>();
+  if (m.width <= 0 || m.height <= 0 || m.width > (1 << 16) || m.height > (1 << 16))
+    fail(R3DG_ERROR_INVALID_ARGUMENT, std::string("corrupt depth map ") + path);
+}
+
 }  // namespace r3dg
 
 using namespace r3dg;
@@ -1531,6 +1639,134 @@ int32_t r3dg_cancel(r3dg_ctx* h, int32_t requested) {
 
 int32_t r3dg_automatic_levels(int32_t width, int32_t height, int32_t num_disparities) {
   return automatic_levels(width, height, num_disparities);
+}
+
+int32_t r3dg_pfm_header(const char* path, int32_t* width, int32_t* height, int32_t* channels) {
+  return guarded(nullptr, [&] {
+    const PfmHeader h = pfm_header(read_bytes(path), path);
+    if (width != nullptr) *width = h.width;
+    if (height != nullptr) *height = h.height;
+    if (channels != nullptr) *channels = h.channels;
+  });
+}
+
+int32_t r3dg_read_pfm(const char* path, float* out, int64_t stride) {
+  return guarded(nullptr, [&] {
+    const std::vector<std::uint8_t> d = read_bytes(path);
+    const PfmHeader h = pfm_header(d, path);
+    const std::size_t row = std::size_t(h.width) * std::size_t(h.channels);
+    if (out == nullptr || stride < static_cast<int64_t>(row)) fail(R3DG_ERROR_INVALID_ARGUMENT, "read_pfm: output too small");
+    const bool swap = !h.little;  // this library runs on little-endian hosts
+    for (int y = 0; y < h.height; ++y) {
+      const std::uint8_t* src = d.data() + h.data + std::size_t(h.height - 1 - y) * row * 4;
+      float* dst = out + std::int64_t(y) * stride;
+      for (std::size_t i = 0; i < row; ++i) {
+        std::uint32_t u;
+        std::memcpy(&u, src + 4 * i, 4);
+        if (swap) u = ((u & 0xFFU) << 24) | ((u & 0xFF00U) << 8) | ((u >> 8) & 0xFF00U) | (u >> 24);
+        std::memcpy(dst + i, &u, 4);
+      }
+    }
+  });
+}
+
+int32_t r3dg_write_pfm(const char* path, const float* data, int32_t width, int32_t height, int32_t channels,
+                       int64_t stride) {
+  return guarded(nullptr, [&] {
+    if (channels != 1 && channels != 3)
+      fail(R3DG_ERROR_INVALID_ARGUMENT, "write_pfm: " + std::to_string(channels) + " channels (1 or 3 supported)");
+    if (data == nullptr || width <= 0 || height <= 0 || stride < int64_t(width) * channels)
+      fail(R3DG_ERROR_INVALID_ARGUMENT, "write_pfm: invalid image");
+    const std::string header =
+        std::string(channels == 1 ? "Pf" : "PF") + "\n" + std::to_string(width) + " " + std::to_string(height) + "\n-1\n";
+    std::vector<std::uint8_t> out(header.begin(), header.end());
+    const std::size_t row = std::size_t(width) * std::size_t(channels) * 4;
+    out.reserve(out.size() + row * std::size_t(height));
+    for (int y = height - 1; y >= 0; --y) {
+      const auto* r = reinterpret_cast<const std::uint8_t*>(data + std::int64_t(y) * stride);
+      out.insert(out.end(), r, r + row);
+    }
+    write_bytes(path, out);
+  });
+}
+
+int32_t r3dg_read_depth_map_header(const char* path, r3dg_depth_map* map) {
+  return guarded(nullptr, [&] {
+    if (map == nullptr) fail(R3DG_ERROR_INVALID_ARGUMENT, "null depth map");
+    const std::vector<std::uint8_t> d = read_bytes(path);
+    ByteReader r{d};
+    try {
+      depth_header(r, *map, path);
+    } catch (const std::out_of_range&) {
+      fail(R3DG_ERROR_INVALID_ARGUMENT, std::string("truncated depth map ") + path);
+    }
+  });
+}
+
+int32_t r3dg_read_depth_map(const char* path, r3dg_depth_map* map) {
+  return guarded(nullptr, [&] {
+    if (map == nullptr || map->depth == nullptr || map->confidence == nullptr || map->pair_mask == nullptr ||
+        map->support == nullptr || map->color == nullptr)
+      fail(R3DG_ERROR_INVALID_ARGUMENT, "read_depth_map: missing image buffers");
+    const std::vector<std::uint8_t> d = read_bytes(path);
+    ByteReader r{d};
+    r3dg_depth_map h = *map;
+    try {
+      depth_header(r, h, path);
+      if (h.width != map->width || h.height != map->height)
+        fail(R3DG_ERROR_INVALID_ARGUMENT, "read_depth_map: buffers sized for another grid");
+      const std::size_t n = std::size_t(h.width) * std::size_t(h.height);
+      r.bytes(map->depth, 4 * n);
+      r.bytes(map->confidence, 4 * n);
+      r.bytes(map->pair_mask, 2 * n);
+      r.bytes(map->support, n);
+      r.bytes(map->color, 3 * n);
+    } catch (const std::out_of_range&) {
+      fail(R3DG_ERROR_INVALID_ARGUMENT, std::string("truncated depth map ") + path);
+    }
+    h.depth = map->depth;
+    h.confidence = map->confidence;
+    h.pair_mask = map->pair_mask;
+    h.support = map->support;
+    h.color = map->color;
+    *map = h;
+  });
+}
+
+int32_t r3dg_write_depth_map(const char* path, const r3dg_depth_map* map) {
+  return guarded(nullptr, [&] {
+    if (map == nullptr || map->num_partners > 16 || map->width <= 0 || map->height <= 0 || map->depth == nullptr ||
+        map->confidence == nullptr || map->pair_mask == nullptr || map->support == nullptr || map->color == nullptr)
+      fail(R3DG_ERROR_INVALID_ARGUMENT, "write_depth_map: invalid depth map");
+    std::vector<std::uint8_t> out;
+    auto pod = [&](const auto& v) {
+      const auto* b = reinterpret_cast<const std::uint8_t*>(&v);
+      out.insert(out.end(), b, b + sizeof(v));
+    };
+    auto raw = [&](const void* p, std::size_t n) {
+      const auto* b = static_cast<const std::uint8_t*>(p);
+      out.insert(out.end(), b, b + n);
+    };
+    raw(kDepthMagic, 4);
+    pod(kDepthVersion);
+    pod(map->view);
+    pod(map->camera_width);
+    pod(map->camera_height);
+    for (const double v : {map->fx, map->fy, map->cx, map->cy}) pod(v);
+    for (const double v : map->rotation) pod(v);
+    for (const double v : map->center) pod(v);
+    pod(map->num_partners);
+    for (std::uint32_t i = 0; i < map->num_partners; ++i) pod(map->partners[i]);
+    pod(map->width);
+    pod(map->height);
+    const std::size_t n = std::size_t(map->width) * std::size_t(map->height);
+    raw(map->depth, 4 * n);
+    raw(map->confidence, 4 * n);
+    raw(map->pair_mask, 2 * n);
+    raw(map->support, n);
+    raw(map->color, 3 * n);
+    write_bytes(path, out);
+  });
 }
 
 int32_t r3dg_combine_pairs(r3dg_ctx* h, const r3dg_grid* grid, const r3dg_pair_geometry* pairs, int32_t count,

@@ -106,6 +106,66 @@ def _shape(a):
     return (int(a.shape[0]), int(a.shape[1]))
 
 
+def _global_check(rc):
+    if rc != 0:
+        buf = ctypes.create_string_buffer(4096)
+        lib().r3dg_global_error(buf, ctypes.c_size_t(len(buf)))
+        raise R3dgError(rc, buf.value.decode(errors="replace"))
+
+
+# ---- file formats (io/pfm.hpp, mvs/depth_map.hpp); host only
+def read_pfm(path):
+    """read_pfm: (h, w) float32, or (h, w, 3) for a colour PFM."""
+    w, h, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    p = os.fsencode(path)
+    _global_check(lib().r3dg_pfm_header(p, ctypes.byref(w), ctypes.byref(h), ctypes.byref(c)))
+    out = np.empty((h.value, w.value * c.value), np.float32)
+    _global_check(lib().r3dg_read_pfm(p, out.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(out.shape[1])))
+    return out if c.value == 1 else out.reshape(h.value, w.value, 3)
+
+
+def write_pfm(path, img):
+    """write_pfm of a (h, w) or (h, w, 3) float image."""
+    a = np.ascontiguousarray(img, np.float32)
+    ch = 1 if a.ndim == 2 else a.shape[2]
+    _global_check(lib().r3dg_write_pfm(os.fsencode(path), a.ctypes.data_as(ctypes.c_void_p), a.shape[1], a.shape[0],
+                                       ch, ctypes.c_int64(a.shape[1] * ch)))
+
+
+def read_depth_map(path):
+    """read_depth_map (.r3dd, version 2) into a DepthMapFile."""
+    from ._abi import CDepthMap, DepthMapFile
+    p = os.fsencode(path)
+    m = CDepthMap()
+    _global_check(lib().r3dg_read_depth_map_header(p, ctypes.byref(m)))
+    h, w = m.height, m.width
+    imgs = (np.empty((h, w), np.float32), np.empty((h, w), np.float32), np.empty((h, w), np.uint16),
+            np.empty((h, w), np.uint8), np.empty((h, w, 3), np.uint8))
+    m.depth, m.confidence, m.pair_mask, m.support, m.color = [a.ctypes.data for a in imgs]
+    _global_check(lib().r3dg_read_depth_map(p, ctypes.byref(m)))
+    return DepthMapFile(m.view, m.camera_width, m.camera_height, m.fx, m.fy, m.cx, m.cy, tuple(m.rotation),
+                        tuple(m.center), tuple(m.partners[:m.num_partners]), *imgs)
+
+
+def write_depth_map(path, dm):
+    """write_depth_map (.r3dd, version 2) of a DepthMapFile."""
+    from ._abi import CDepthMap
+    h, w = np.asarray(dm.depth).shape
+    imgs = [np.ascontiguousarray(dm.depth, np.float32), np.ascontiguousarray(dm.confidence, np.float32),
+            np.ascontiguousarray(dm.pair_mask, np.uint16), np.ascontiguousarray(dm.support, np.uint8),
+            np.ascontiguousarray(dm.color, np.uint8)]
+    if imgs[4].shape != (h, w, 3) or any(a.shape != (h, w) for a in imgs[:4]):
+        raise ValueError("write_depth_map: images of different sizes")
+    if len(dm.partners) > 16:
+        raise ValueError("write_depth_map: at most 16 partners")
+    parts = list(dm.partners) + [0] * (16 - len(dm.partners))
+    m = CDepthMap(int(dm.view), int(dm.camera_width), int(dm.camera_height), len(dm.partners), float(dm.fx),
+                  float(dm.fy), float(dm.cx), float(dm.cy), (ctypes.c_double * 9)(*[float(v) for v in dm.rotation]),
+                  (ctypes.c_double * 3)(*[float(v) for v in dm.center]), (ctypes.c_uint32 * 16)(*parts), w, h,
+                  *[a.ctypes.data for a in imgs])
+    _global_check(lib().r3dg_write_depth_map(os.fsencode(path), ctypes.byref(m)))
+
+
 class Matcher:
     """One context on one GPU (r3dg_ctx); the methods mirror the reference API."""
 
