@@ -401,7 +401,7 @@ __device__ __forceinline__ void stb(void* base, std::uint32_t off, T v) {
 // operation and nothing else: vmcnt 0, expcnt 7, lgkmcnt 15.
 constexpr int kVmcnt0 = 0x0F70;
 constexpr int kWaves = 4;                     // waves per block
-constexpr int kDist = 2;                      // steps between a segment prefetch and its use
+constexpr int kDist = 3;                      // steps between a segment prefetch and its use
 constexpr int kNarrow = 368;                  // labels whose path costs stay in LDS
 constexpr int kPad = 12;                      // sentinel slots before label 0 and after the last label
 constexpr int kRow = kPad + kNarrow + kPad;   // uint16 slots of an LDS row
@@ -423,11 +423,26 @@ struct SegIn {
   std::uint32_t s[P / 2];
 };
 
-// Per-step values of one scanline pixel, loaded two steps ahead.
+// Per-step values of one scanline pixel, read from the wave's step ring:
+// the pixel's first cell in the tile volume and a packed word holding its
+// first label (bits 0-15), the P2 table index |I - I_prev| >> 8 of the step
+// (bits 16-23) and its label count / 16 (bits 24-31); p2 is the table
+// entry itself, read two steps before the step.
 struct StepIn {
-  std::uint64_t rec;  // pixel record
-  std::uint32_t rs;   // cell offset of the pixel's band row (band correction)
-  std::uint32_t ip;   // base intensity
+  std::uint32_t cell;
+  std::uint32_t meta;
+  std::uint32_t p2;
+};
+static_assert(kMaxLabels / kLabelAlign <= 255, "label count / 16 must fit 8 bits");
+__device__ __forceinline__ int step_lo(const StepIn& u) { return static_cast<int>(u.meta & 0xFFFFU); }
+__device__ __forceinline__ std::uint32_t step_p2_index(const StepIn& u) { return (u.meta >> 16) & 0xFFU; }
+__device__ __forceinline__ int step_num(const StepIn& u) { return static_cast<int>(u.meta >> 24) * kLabelAlign; }
+// Pixel record, band-row offset and intensity of step 8 b + l of a scanline,
+// held by lane l of its group: the loads of 8 steps are one batch.
+struct StepBatch {
+  std::uint64_t rec;
+  std::uint32_t rs;
+  std::uint32_t ip;
 };
 // Segments 0 and 1 of a pixel, loaded one step ahead.
 struct StepSegs {
@@ -445,8 +460,11 @@ struct StepSegs {
 // most kNarrow labels and in a global scratch row for wider ones; a lane reads
 // its window of P + 2 values as aligned words (the window clamped into the
 // sentinels) and shifts them into place.  Segments 0 and 1 use data prefetched
-// kDist steps ahead (pixel records one step earlier still); segments from 2 on
-// (pixels of more than 96 labels) are loaded in the step, one segment ahead.
+// kDist steps ahead; segments from 2 on (pixels of more than 96 labels) are
+// loaded in the step, one segment ahead.  The pixel records, band-row offsets
+// and intensities of a scanline are loaded 8 steps at a time (lane l of the
+// group loads step 8 b + l) and unpacked into an 8-slot ring in LDS, read one
+// step before the segment prefetch that needs them.
 template <int DX, int DY, bool kAcc>
 __global__ __launch_bounds__(64 * kWaves) void k_path(
     const JobDesc* __restrict__ jobs, const TileDesc* __restrict__ tiles, const std::uint64_t* __restrict__ gprefix,
@@ -455,6 +473,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
     std::uint16_t* __restrict__ scratch, int scratch_row, std::uint32_t* __restrict__ queue) {
   __shared__ __attribute__((aligned(16))) std::uint16_t lut[kWaves][256];
   __shared__ __attribute__((aligned(16))) std::uint32_t rows[kWaves][kScanPerWave][2][kRow / 2];
+  __shared__ __attribute__((aligned(16))) u32x2 ring[kWaves][8][kScanPerWave];
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63, grp = lane / kLanesPerScan, l8 = lane % kLanesPerScan;
   // Resident wave index: the wave keeps its LDS rows and its scratch rows for
@@ -516,18 +535,37 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
   const std::uint8_t* Ct = C + T.cell_base;
   std::uint16_t* St = S + T.cell_base;
 
-  auto load_in = [&](int i, StepIn& u) {
-    i = min(i, len - 1);
+  // Batch of steps [base, base + 8): lane l8 loads step base + l8 (steps past
+  // the scanline repeat its last pixel; they are never active).
+  auto load_batch = [&](int base, StepBatch& b) {
+    const int i = min(base + l8, len - 1);
     const int x = x0 + i * DX, y = y0 + i * DY;
-    u.rec = ldb<std::uint64_t>(tpix, static_cast<std::uint32_t>(y * T.w + x) * 8U);
-    u.rs = ldb<std::uint32_t>(tbc, static_cast<std::uint32_t>((x >> T.band_shift) * T.h + y) * 8U);
-    u.ip = ldb<std::uint16_t>(img, static_cast<std::uint32_t>((T.y0 + y) * img_stride + T.x0 + x) * 2U);
+    b.rec = ldb<std::uint64_t>(tpix, static_cast<std::uint32_t>(y * T.w + x) * 8U);
+    b.rs = ldb<std::uint32_t>(tbc, static_cast<std::uint32_t>((x >> T.band_shift) * T.h + y) * 8U);
+    b.ip = ldb<std::uint16_t>(img, static_cast<std::uint32_t>((T.y0 + y) * img_stride + T.x0 + x) * 2U);
+  };
+  // Unpacks a batch into the ring (slot l8 of the group); last_ip carries the
+  // intensity of the batch's last step to the next batch's first.
+  std::uint32_t last_ip = 0;
+  auto write_batch = [&](const StepBatch& b) {
+    std::uint32_t prev = __shfl_up(b.ip, 1U, kLanesPerScan);
+    prev = l8 == 0 ? last_ip : prev;
+    const std::uint32_t d = static_cast<std::uint32_t>(abs(static_cast<int>(b.ip) - static_cast<int>(prev))) >> 8;
+    ring[wave][l8][grp] = u32x2{b.rs + pix_off(b.rec), static_cast<std::uint32_t>(pix_lo(b.rec)) | d << 16 |
+                                                           static_cast<std::uint32_t>(pix_num(b.rec) / kLabelAlign) << 24};
+    last_ip = __shfl(b.ip, kLanesPerScan - 1, kLanesPerScan);
+    __builtin_amdgcn_wave_barrier();
+  };
+  auto read_step = [&](int i, StepIn& u) {
+    const u32x2 e = ring[wave][i & 7][grp];
+    u.cell = e.x;
+    u.meta = e.y;
   };
   // Costs and S of segment s (lanes past the pixel's labels read its last ones;
   // those values are never used).
   auto load_seg4 = [&](const StepIn& u, SegIn<4>& d) {
-    const std::uint32_t cell = u.rs + pix_off(u.rec);
-    const std::uint32_t j0 = static_cast<std::uint32_t>(min(4 * l8, pix_num(u.rec) - 4));
+    const std::uint32_t cell = u.cell;
+    const std::uint32_t j0 = static_cast<std::uint32_t>(min(4 * l8, step_num(u) - 4));
     d.c[0] = ldb<std::uint32_t>(Ct, cell + j0);
     if constexpr (kAcc) {
       const u32x2 v = ldb<u32x2>(St, (cell + j0) * 2U);
@@ -536,8 +574,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
     }
   };
   auto load_seg8 = [&](const StepIn& u, int s, SegIn<8>& d) {
-    const std::uint32_t cell = u.rs + pix_off(u.rec);
-    const std::uint32_t j0 = static_cast<std::uint32_t>(min(seg_first<8>(s, l8), pix_num(u.rec) - 8));
+    const std::uint32_t cell = u.cell;
+    const std::uint32_t j0 = static_cast<std::uint32_t>(min(seg_first<8>(s, l8), step_num(u) - 8));
     const u32x2 c = ldb<u32x2>(Ct, cell + j0);
     d.c[0] = c.x;
     d.c[1] = c.y;
@@ -553,25 +591,33 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
   // pixel has more than 32 labels); the step tests the same condition.
   auto load_segs = [&](const StepIn& u, StepSegs& d) {
     load_seg4(u, d.a);
-    if (__builtin_amdgcn_ballot_w64(grp_active && pix_num(u.rec) > 32) != 0) load_seg8(u, 1, d.b);
+    if (__builtin_amdgcn_ballot_w64(grp_active && step_num(u) > 32) != 0) load_seg8(u, 1, d.b);
   };
 
   int prev_lo = 0, prev_num = 0, flip = 0;
-  std::uint32_t prev_ip = 0, prev_min = 0;
-  // Step i: u, v are its pixel values and segments; it loads the pixel values
-  // of step i + kDist + 1 into u_load and the segments of step i + kDist
-  // (pixel values u_seg) into v_load.
-  auto step = [&](int i, const StepIn& u, const StepSegs& v, StepIn& u_load, const StepIn& u_seg,
+  std::uint32_t prev_min = 0;
+  StepBatch B;
+  // Step i: u, v are its pixel values and segments; it reads the pixel values
+  // of step i + kDist + 1 into u_load (first unpacking the batch of that step
+  // and loading the next one when it starts a batch), the P2 entry of step
+  // i + kDist into u_seg and loads that step's segments into v_load.  The
+  // batch of steps [8 b, 8 b + 8) is unpacked at step 8 b - kDist - 1, after
+  // the last read of the slots it overwrites (steps 8 b - 8 .. 8 b - 1, read
+  // by step 8 b - kDist - 2).
+  auto step = [&](int i, const StepIn& u, const StepSegs& v, StepIn& u_load, StepIn& u_seg,
                   StepSegs& v_load) __attribute__((always_inline)) {
-    load_in(i + kDist + 1, u_load);
+    u_seg.p2 = lut[wave][step_p2_index(u_seg)];
+    if (((i + kDist + 1) & 7) == 0) {
+      write_batch(B);
+      load_batch(i + kDist + 1 + 8, B);
+    }
+    read_step(i + kDist + 1, u_load);
     load_segs(u_seg, v_load);
     const bool act = grp_active && i < len;
     const bool first = i == 0;
-    const int lo = pix_lo(u.rec), num = pix_num(u.rec);
-    const std::uint32_t cell = u.rs + pix_off(u.rec);
-    const std::uint32_t pen =
-        min(prev_min + lut[wave][static_cast<std::uint32_t>(abs(static_cast<int>(u.ip) - static_cast<int>(prev_ip))) >> 8],
-            0xFFFFU);
+    const int lo = step_lo(u), num = step_num(u);
+    const std::uint32_t cell = u.cell;
+    const std::uint32_t pen = min(prev_min + u.p2, 0xFFFFU);
     const std::uint32_t PENx2 = pen * 0x10001U, MPx2 = prev_min * 0x10001U;
     const int delta = lo - prev_lo;
     const bool prev_wide = prev_num > kNarrow, cur_wide = num > kNarrow;
@@ -742,7 +788,6 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
     prev_min = min8(min(m & 0xFFFFU, m >> 16));
     prev_lo = lo;
     prev_num = num;
-    prev_ip = u.ip;
     flip ^= 1;
     __builtin_amdgcn_wave_barrier();  // this step's row writes before the next step's reads
   };
@@ -752,7 +797,12 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
   constexpr int NU = kDist + 2, NV = kDist + 1;
   StepIn U[NU];
   StepSegs V[NV];
-  for (int j = 0; j <= kDist; ++j) load_in(j, U[j]);
+  static_assert(kDist + 1 < 8, "the prologue reads the first batch only");
+  load_batch(0, B);
+  write_batch(B);
+  load_batch(8, B);
+  for (int j = 0; j <= kDist; ++j) read_step(j, U[j]);
+  for (int j = 0; j < kDist; ++j) U[j].p2 = lut[wave][step_p2_index(U[j])];
   for (int j = 0; j < kDist; ++j) load_segs(U[j], V[j]);
   auto body = [&](auto Kc, int i) __attribute__((always_inline)) {
     constexpr int K = decltype(Kc)::value;
