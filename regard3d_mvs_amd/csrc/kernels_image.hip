@@ -202,14 +202,14 @@ __global__ void k_apply_mask(float* __restrict__ map, std::size_t n, const std::
 // |d_j - d_i| <= max_diff (the relation is symmetric, so the segments do not
 // depend on the visiting order).  Here: union-find in three steps, every
 // parent pointer only ever moving to a smaller pixel index (no cycles):
-//   1. each 32x8 block unions its pixels in LDS and points them to the
+//   1. each 32x32 block unions its pixels in LDS and points them to the
 //      block-local root;
 //   2. pixels on block borders union across them in HBM (atomicMin links,
 //      Playne & Hawick 2018, with path halving);
 //   3. every pixel finds its root and counts it, saturating at min_size (only
 //      "fewer than min_size" matters, and big segments would serialise on one
 //      counter), then small segments are invalidated.
-constexpr int kCcW = 32, kCcH = 8;
+constexpr int kCcW = 32, kCcH = 32;
 
 __device__ __forceinline__ int ld_relaxed(int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -267,7 +267,7 @@ __device__ void lds_union(int* lab, int a, int b) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_cc_local(const float* __restrict__ d, int w, int h, float max_diff,
+__global__ __launch_bounds__(1024) void k_cc_local(const float* __restrict__ d, int w, int h, float max_diff,
                                                   int* __restrict__ p, int* __restrict__ size) {
   __shared__ int lab[kCcW * kCcH];
   __shared__ float dv[kCcW * kCcH];
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(256) void k_cc_local(const float* __restrict__ d, i
   }
 }
 
-__global__ __launch_bounds__(256) void k_cc_border(const float* __restrict__ d, int w, int h, float max_diff, int* p) {
+__global__ __launch_bounds__(1024) void k_cc_border(const float* __restrict__ d, int w, int h, float max_diff, int* p) {
   const int tx = threadIdx.x, ty = threadIdx.y;
   const int x = blockIdx.x * kCcW + tx, y = blockIdx.y * kCcH + ty;
   if (x >= w || y >= h) return;

@@ -104,49 +104,80 @@ __device__ __forceinline__ U block_scan_incl(U v, U* wsum, U& total) {
   return v + off;
 }
 
-// Exclusive scan of the label counts of one tile row (block-wide, in chunks
-// of 256 pixels) into the pixel records: cell offset inside the row, lo, num.
+// Inclusive scan of v over the 64 lanes of a wave.
+__device__ __forceinline__ std::uint32_t wave_scan_incl(std::uint32_t v) {
+  const int lane = static_cast<int>(threadIdx.x & 63);
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const std::uint32_t t = __shfl_up(v, static_cast<unsigned>(d));
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan of the label counts of one tile row into the pixel records:
+// cell offset inside the row, lo, num.  One wave per row (4 rows per block),
+// 64 x kItems pixels per pass, the pass's label ranges loaded before its
+// scans; no block barriers, so a row waits only on its own loads.
 __global__ __launch_bounds__(256) void k_tile_row_scan(const JobDesc* __restrict__ jobs,
                                                        const TileDesc* __restrict__ tiles,
                                                        const std::uint64_t* __restrict__ row_prefix, int ntiles,
+                                                       std::uint64_t total_rows,
                                                        std::uint64_t* __restrict__ pix,
                                                        std::uint64_t* __restrict__ row_cells,
                                                        std::uint32_t* __restrict__ row_max,
                                                        std::uint64_t* __restrict__ bstart) {
-  const std::uint64_t r = blockIdx.x;
+  constexpr int kItems = 8;
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const std::uint64_t r = std::uint64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (r >= total_rows) return;  // whole waves
   const int t = find_tile_wave(row_prefix, ntiles, r);
   const TileDesc T = tiles[t];
   const JobDesc& J = jobs[T.job];
   const int y = static_cast<int>(r - row_prefix[t]);
   const int Y = T.y0 + y;
-  __shared__ std::uint32_t wsum[4];
-  __shared__ std::uint32_t smax;
-  if (threadIdx.x == 0) smax = 0;
-  __syncthreads();
-  std::uint32_t carry = 0;
-  for (int base = 0; base < T.w; base += 256) {
-    const int x = base + threadIdx.x;
-    std::uint32_t v = 0;
-    int lo = 0, num = 0;
-    if (x < T.w) {
-      label_range(J, T.x0 + x, Y, lo, num);
-      v = static_cast<std::uint32_t>(num);
-      atomicMax(&smax, v);
+  const auto jlo = gptr(J.lo), jnum = gptr(J.num);
+  const std::int64_t rbase = std::int64_t(Y) * J.r_stride + T.x0;
+  const int bmask = (1 << min(T.band_shift, 30)) - 1;
+  std::uint64_t* prow = pix + T.px_base + std::uint64_t(y) * T.w;
+  std::uint32_t carry = 0, vmax = 0;
+  for (int base = 0; base < T.w; base += 64 * kItems) {
+    int lo[kItems], num[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+      const int x = base + 64 * k + lane;
+      lo[k] = 0;
+      num[k] = 0;
+      if (x < T.w) {
+        if (jlo != nullptr) {
+          lo[k] = jlo[rbase + x];
+          num[k] = jnum[rbase + x];
+        } else {
+          num[k] = J.D;
+        }
+      }
     }
-    std::uint32_t total;
-    const std::uint32_t incl = block_scan_incl(v, wsum, total);
-    if (x < T.w) {
-      const std::uint32_t off = carry + incl - v;
-      pix[T.px_base + std::uint64_t(y) * T.w + x] = pix_record(off, lo, num);
-      // Row offset of the first pixel of each band.
-      if ((x & ((1 << min(T.band_shift, 30)) - 1)) == 0 && (x >> T.band_shift) < T.nbands)
-        bstart[T.band_base + std::uint64_t(x >> T.band_shift) * T.h + y] = off;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+      const int x = base + 64 * k + lane;
+      const std::uint32_t v = static_cast<std::uint32_t>(num[k]);
+      const std::uint32_t incl = wave_scan_incl(v);
+      if (x < T.w) {
+        const std::uint32_t off = carry + incl - v;
+        prow[x] = pix_record(off, lo[k], num[k]);
+        // Row offset of the first pixel of each band.
+        if ((x & bmask) == 0 && (x >> T.band_shift) < T.nbands)
+          bstart[T.band_base + std::uint64_t(x >> T.band_shift) * T.h + y] = off;
+      }
+      carry += static_cast<std::uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 63));
+      vmax = max(vmax, v);
     }
-    carry += total;
   }
-  if (threadIdx.x == 0) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) vmax = max(vmax, static_cast<std::uint32_t>(__shfl_xor(vmax, d)));
+  if (lane == 0) {
     row_cells[T.row_base + y] = carry;
-    row_max[T.row_base + y] = smax;
+    row_max[T.row_base + y] = vmax;
   }
 }
 
@@ -1178,8 +1209,10 @@ void path_launch(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t
 void launch_tile_row_scan(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* row_prefix, int ntiles,
                           std::uint64_t total_rows, std::uint64_t* pix, std::uint64_t* row_cells,
                           std::uint32_t* row_max, std::uint64_t* bstart, hipStream_t s) {
-  k_tile_row_scan<<<dim3(static_cast<unsigned>(total_rows)), 256, 0, s>>>(jobs, tiles, row_prefix, ntiles, pix,
-                                                                          row_cells, row_max, bstart);
+  if (total_rows == 0) return;
+  k_tile_row_scan<<<dim3(static_cast<unsigned>((total_rows + 3) / 4)), 256, 0, s>>>(jobs, tiles, row_prefix, ntiles,
+                                                                                    total_rows, pix, row_cells, row_max,
+                                                                                    bstart);
 }
 
 void launch_tile_cells(const TileDesc* tiles, int ntiles, const std::uint64_t* row_cells, const std::uint32_t* row_max,
