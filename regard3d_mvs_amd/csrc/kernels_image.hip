@@ -275,12 +275,23 @@ __global__ __launch_bounds__(1024) void k_cc_local(const float* __restrict__ d, 
   const int x = blockIdx.x * kCcW + tx, y = blockIdx.y * kCcH + ty;
   const bool in = x < w && y < h;
   dv[t] = in ? d[std::int64_t(y) * w + x] : __builtin_nanf("");
-  lab[t] = t;
   __syncthreads();
-  if (in) {
-    const float v = dv[t];
-    if (tx + 1 < kCcW && x + 1 < w && fabsf(dv[t + 1] - v) <= max_diff) lds_union(lab, t, t + 1);
-    if (ty + 1 < kCcH && y + 1 < h && fabsf(dv[t + kCcW] - v) <= max_diff) lds_union(lab, t, t + kCcW);
+  // Runs of horizontally joined pixels first: a wave holds two block rows,
+  // and a pixel's parent is the first pixel of its run (the last lane at or
+  // before it that does not join its left neighbour; lane 0 of each row
+  // never does).
+  const float v = dv[t];
+  const bool left = in && tx > 0 && fabsf(v - dv[t - 1]) <= max_diff;
+  const int lane = t & 63;
+  const std::uint64_t starts = __builtin_amdgcn_ballot_w64(!left) & (~0ULL >> (63 - lane));
+  lab[t] = t - lane + (63 - __builtin_clzll(starts));
+  __syncthreads();
+  // Vertical joins, skipped where the pixels to the left are joined to these
+  // two and to each other (that union already holds both runs).
+  if (in && ty + 1 < kCcH && y + 1 < h && fabsf(dv[t + kCcW] - v) <= max_diff) {
+    const bool implied = left && fabsf(dv[t + kCcW] - dv[t + kCcW - 1]) <= max_diff &&
+                         fabsf(dv[t + kCcW - 1] - dv[t - 1]) <= max_diff;
+    if (!implied) lds_union(lab, t, t + kCcW);
   }
   __syncthreads();
   if (in) {
