@@ -473,7 +473,7 @@ __device__ __forceinline__ int step_num(const StepIn& u) { return static_cast<in
 struct StepBatch {
   std::uint64_t rec;
   std::uint32_t rs;
-  std::uint32_t ip;
+  std::uint16_t ip;  // widened where it is used, so no wait follows the load
 };
 // Segments 0 and 1 of a pixel, loaded one step ahead.
 struct StepSegs {
@@ -586,6 +586,9 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
                                                            static_cast<std::uint32_t>(pix_num(b.rec) / kLabelAlign) << 24};
     last_ip = __shfl(b.ip, kLanesPerScan - 1, kLanesPerScan);
     __builtin_amdgcn_wave_barrier();
+    // Keep the shuffles of b ahead of the loads that refill it: otherwise the
+    // new intensity lands in a temporary whose copy into b waits for the load.
+    __builtin_amdgcn_sched_barrier(0);
   };
   auto read_step = [&](int i, StepIn& u) {
     const u32x2 e = ring[wave][i & 7][grp];
