@@ -28,39 +28,60 @@ inline dim3 grid2(int w, int h, dim3 b) { return dim3((w + b.x - 1) / b.x, (h + 
 // replicated (census.cpp:137-155).  A 64x4 block stages its (64+8)x(4+6) u16
 // halo in LDS with coalesced row loads and writes one u64 per thread.
 constexpr int kCw = 64, kCh = 4;
+// A thread computes two horizontally adjacent pixels: a block covers
+// (2 kCw) x kCh pixels, and each row of the two 9-wide windows (10 columns)
+// is five aligned 32-bit LDS reads.
 __global__ __launch_bounds__(256) void k_census(const std::uint16_t* __restrict__ img, std::int64_t is, int w, int h,
                                                 std::uint64_t* __restrict__ out, std::int64_t os) {
-  __shared__ std::uint16_t t[kCh + 6][kCw + 8];
-  const int bx = blockIdx.x * kCw, by = blockIdx.y * kCh;
+  constexpr int kTw = 2 * kCw + 8;  // staged columns (even: rows stay 4-byte aligned)
+  __shared__ __attribute__((aligned(4))) std::uint16_t t[kCh + 6][kTw];
+  const int bx = blockIdx.x * 2 * kCw, by = blockIdx.y * kCh;
   const int tid = threadIdx.y * kCw + threadIdx.x;
-  for (int i = tid; i < (kCh + 6) * (kCw + 8); i += 256) {
-    const int ly = i / (kCw + 8), lx = i - ly * (kCw + 8);
+  for (int i = tid; i < (kCh + 6) * kTw; i += 256) {
+    const int ly = i / kTw, lx = i - ly * kTw;
     const int gy = min(max(by + ly - 3, 0), h - 1);
     const int gx = min(max(bx + lx - 4, 0), w - 1);
     t[ly][lx] = img[gy * is + gx];
   }
   __syncthreads();
-  const int x = bx + threadIdx.x, y = by + threadIdx.y;
+  const int x = bx + 2 * threadIdx.x, y = by + threadIdx.y;
   if (x >= w || y >= h) return;
   const int tx = threadIdx.x, ty = threadIdx.y;
-  const std::uint16_t c = t[ty + 3][tx + 4];
-  std::uint32_t hi = 0, lo = 0;  // bits 61..32 (30 neighbours), 31..0 (32 neighbours)
+  // Columns 2 tx .. 2 tx + 9 of window row j: pixel x uses 0..8, x + 1 uses 1..9.
+  auto row = [&](int j, std::uint32_t* v) {
+    const std::uint32_t* r = reinterpret_cast<const std::uint32_t*>(&t[ty + j][0]) + tx;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const std::uint32_t u = r[k];
+      v[2 * k] = u & 0xFFFFU;
+      v[2 * k + 1] = u >> 16;
+    }
+  };
+  std::uint32_t c[10];
+  row(3, c);
+  const std::uint32_t ca = c[4], cb = c[5];
+  std::uint32_t ahi = 0, alo = 0, bhi = 0, blo = 0;  // bits 61..32 and 31..0
   int k = 0;
 #pragma unroll
   for (int j = 0; j < 7; ++j) {
+    std::uint32_t v[10];
+    row(j, v);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       if (j == 3 && i == 4) continue;
-      const std::uint32_t b = t[ty + j][tx + i] < c ? 1U : 0U;
+      const std::uint32_t a = v[i] < ca ? 1U : 0U, b = v[i + 1] < cb ? 1U : 0U;
       if (k < 30) {
-        hi = (hi << 1) | b;
+        ahi = (ahi << 1) | a;
+        bhi = (bhi << 1) | b;
       } else {
-        lo = (lo << 1) | b;
+        alo = (alo << 1) | a;
+        blo = (blo << 1) | b;
       }
       ++k;
     }
   }
-  out[y * os + x] = (std::uint64_t(hi) << 32) | lo;
+  out[y * os + x] = (std::uint64_t(ahi) << 32) | alo;
+  if (x + 1 < w) out[y * os + x + 1] = (std::uint64_t(bhi) << 32) | blo;
 }
 
 // ------------------------------------------------------------------ pyramid
@@ -782,7 +803,7 @@ int range_levels(int wc, int hc, int fill, int* ws, int* hs) {
 void launch_census(const std::uint16_t* img, std::int64_t is, int w, int h, std::uint64_t* out, std::int64_t os,
                    hipStream_t s) {
   const dim3 b(kCw, kCh);
-  k_census<<<grid2(w, h, b), b, 0, s>>>(img, is, w, h, out, os);
+  k_census<<<dim3((w + 2 * kCw - 1) / (2 * kCw), (h + kCh - 1) / kCh), b, 0, s>>>(img, is, w, h, out, os);
 }
 
 void launch_pyramid_u16(const std::uint16_t* in, std::int64_t is, int w, int h, std::uint32_t* tmp,

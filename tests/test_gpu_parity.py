@@ -117,6 +117,26 @@ def test_cost_volume_full_range_512(gpu):
     same_bits(gpu.cost_volume(p.left, p.right, 0, 512), O.cost_volume(p.left, p.right, 0, 512))
 
 
+def test_cost_volume_max_labels(gpu):
+    # 4000 labels, the most a pixel may hold on the GPU: label counts up to
+    # 4000 (count / 16 = 250) and first labels up to 3984 go through the path
+    # scans' packed per-step records; every path cost lives in scratch rows.
+    w, h, nd = 22, 5, 4000
+    rng = np.random.default_rng(77)
+    left = rng.integers(0, 65536, size=(h, w)).astype(np.uint16)
+    right = rng.integers(0, 65536, size=(h, w + 30)).astype(np.uint16)
+    o = StereoOptions(p2=300, p2_adapt=6)
+    same_bits(gpu.cost_volume(left, right, -4010, nd, o), O.cost_volume(left, right, -4010, nd, o))
+    num = (16 * rng.integers(1, 251, size=(h, w))).astype(np.uint16)
+    lo = (rng.random((h, w)) * (nd - num + 1)).astype(np.uint16)
+    num[0, :3] = nd
+    lo[0, :3] = 0
+    lo[1, :3] = nd - 16
+    num[1, :3] = 16
+    same_bits(gpu.cost_volume(left, right, -4010, nd, o, (lo, num)),
+              O.cost_volume(left, right, -4010, nd, o, (lo, num)))
+
+
 def test_cost_volume_mixed_wide_ranges(gpu):
     # Label counts from 16 to 512 in random order along every path: the path
     # costs of a pixel move between LDS rows (narrow) and scratch rows (wide)
