@@ -458,7 +458,7 @@ struct SegIn {
 // the pixel's first cell in the tile volume and a packed word holding its
 // first label (bits 0-15), the P2 table index |I - I_prev| >> 8 of the step
 // (bits 16-23) and its label count / 16 (bits 24-31); p2 is the table
-// entry itself, read two steps before the step.
+// entry itself, read kDist steps before the step.
 struct StepIn {
   std::uint32_t cell;
   std::uint32_t meta;
