@@ -231,6 +231,7 @@ __global__ void k_apply_mask(float* __restrict__ map, std::size_t n, const std::
 //      "fewer than min_size" matters, and big segments would serialise on one
 //      counter), then small segments are invalidated.
 constexpr int kCcW = 32, kCcH = 32;
+static_assert(64 % kCcW == 0, "the local phase labels runs per wave: block rows must not straddle waves");
 
 __device__ __forceinline__ int ld_relaxed(int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
