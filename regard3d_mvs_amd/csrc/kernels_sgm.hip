@@ -625,7 +625,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
   // pixel has more than 32 labels); the step tests the same condition.
   auto load_segs = [&](const StepIn& u, StepSegs& d) {
     load_seg4(u, d.a);
-    if (__builtin_amdgcn_ballot_w64(grp_active && step_num(u) > 32) != 0) load_seg8(u, 1, d.b);
+    if (__builtin_amdgcn_ballot_w64((grp_active ? step_num(u) : 0) > 32) != 0) load_seg8(u, 1, d.b);
   };
 
   int prev_lo = 0, prev_num = 0, flip = 0;
@@ -656,10 +656,12 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
     const int delta = lo - prev_lo;
     const bool prev_wide = prev_num > kNarrow, cur_wide = num > kNarrow;
     const int n_lds = min(prev_num, kNarrow);
-    const bool any_seg1 = __builtin_amdgcn_ballot_w64(act && num > 32) != 0;
-    const bool any_seg2 = __builtin_amdgcn_ballot_w64(act && num > 96) != 0;
-    const bool any_prev_wide = __builtin_amdgcn_ballot_w64(act && prev_wide) != 0;
-    const bool any_cur_wide = __builtin_amdgcn_ballot_w64(act && cur_wide) != 0;
+    // Label counts of active lanes (0 elsewhere): each ballot is one compare.
+    const int num_act = act ? num : 0, prev_act = act ? prev_num : 0;
+    const bool any_seg1 = __builtin_amdgcn_ballot_w64(num_act > 32) != 0;
+    const bool any_seg2 = __builtin_amdgcn_ballot_w64(num_act > 96) != 0;
+    const bool any_prev_wide = __builtin_amdgcn_ballot_w64(prev_act > kNarrow) != 0;
+    const bool any_cur_wide = __builtin_amdgcn_ballot_w64(num_act > kNarrow) != 0;
     std::uint32_t* cur = rows[wave][grp][flip];
     const std::uint32_t* prv = rows[wave][grp][flip ^ 1];
     __attribute__((address_space(1))) std::uint32_t* scur = srow + flip * swords;
