@@ -1104,12 +1104,14 @@ __device__ __forceinline__ float axis_weight(int v, int c0, int c1, int size, in
   return w;
 }
 
-// First grid index whose extent end exceeds v (extents sorted by start and end).
-__device__ __forceinline__ int first_covering(const std::int32_t* e, int n, int v) {
+// First grid index whose extent end exceeds v (extents sorted by start and
+// end); e(k) is extent k's end.
+template <typename E>
+__device__ __forceinline__ int first_covering(E e, int n, int v) {
   int lo = 0, hi = n;
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
-    if (e[4 * mid + 1] > v) {
+    if (e(mid) > v) {
       hi = mid;
     } else {
       lo = mid + 1;
@@ -1121,37 +1123,58 @@ __device__ __forceinline__ int first_covering(const std::int32_t* e, int n, int 
 // Blend of one job's tiles (stereo_matcher.cpp:688-766): per pixel, the tiles
 // that cover it in tile-index order, the same float operations as the
 // reference (weighted mean if the valid weight is at least half of the total,
-// the dominant tile's value if the mean is more than 1 away from it).
-__global__ void k_blend(GridDesc g, const TileDesc* __restrict__ tiles, const float* __restrict__ tdisp,
-                        const float* __restrict__ tconf, const std::int16_t* __restrict__ tlab,
-                        float* __restrict__ disp, float* __restrict__ conf, std::int16_t* __restrict__ lab) {
+// the dominant tile's value if the mean is more than 1 away from it).  The
+// block copies the grid's extents into LDS first (grids of at most kGrid
+// tiles per axis), so a pixel's searches wait on one coalesced load.
+__global__ __launch_bounds__(256) void k_blend(GridDesc g, const TileDesc* __restrict__ tiles,
+                                               const float* __restrict__ tdisp, const float* __restrict__ tconf,
+                                               const std::int16_t* __restrict__ tlab, float* __restrict__ disp,
+                                               float* __restrict__ conf, std::int16_t* __restrict__ lab) {
+  constexpr int kGrid = 256;
+  __shared__ std::int32_t scols[4 * kGrid], srows[4 * kGrid];
+  const bool staged = g.nx <= kGrid && g.ny <= kGrid;
+  if (staged) {
+    const auto gc = gptr(g.cols), gr = gptr(g.rows);
+    for (int k = threadIdx.x; k < 4 * g.nx; k += blockDim.x) scols[k] = gc[k];
+    for (int k = threadIdx.x; k < 4 * g.ny; k += blockDim.x) srows[k] = gr[k];
+  }
+  __syncthreads();
   const int X = blockIdx.x * blockDim.x + threadIdx.x, Y = blockIdx.y;
   if (X >= g.width) return;
   float wall = 0.0F, wsum = 0.0F, dsum = 0.0F, csum = 0.0F, wbest = 0.0F, dbest = kInf;
   std::int16_t lbest = INT16_MIN;
-  const int j0 = first_covering(g.rows, g.ny, Y);
-  const int i0 = first_covering(g.cols, g.nx, X);
-  for (int j = j0; j < g.ny && g.rows[4 * j] <= Y; ++j) {
-    const float wy = axis_weight(Y, g.rows[4 * j + 2], g.rows[4 * j + 3], g.height, g.blend);
-    if (wy <= 0.0F) continue;
-    for (int i = i0; i < g.nx && g.cols[4 * i] <= X; ++i) {
-      const float wgt = wy * axis_weight(X, g.cols[4 * i + 2], g.cols[4 * i + 3], g.width, g.blend);
-      if (wgt <= 0.0F) continue;
-      const TileDesc& T = tiles[g.tile_base + j * g.nx + i];
-      const std::uint64_t ti = T.px_base + std::uint64_t(Y - T.y0) * T.w + (X - T.x0);
-      wall += wgt;
-      const float dv = tdisp[ti];
-      if (__builtin_isfinite(dv)) {
-        wsum += wgt;
-        dsum += wgt * dv;
-        if (conf != nullptr) csum += wgt * tconf[ti];
-        if (wgt > wbest) {
-          wbest = wgt;
-          dbest = dv;
-          if (lab != nullptr) lbest = tlab[ti];
+  // cols(i, k) / rows(j, k): entry k of extent i / j.
+  auto blend = [&](auto cols, auto rows) {
+    const int j0 = first_covering([&](int k) { return rows(k, 1); }, g.ny, Y);
+    const int i0 = first_covering([&](int k) { return cols(k, 1); }, g.nx, X);
+    for (int j = j0; j < g.ny && rows(j, 0) <= Y; ++j) {
+      const float wy = axis_weight(Y, rows(j, 2), rows(j, 3), g.height, g.blend);
+      if (wy <= 0.0F) continue;
+      for (int i = i0; i < g.nx && cols(i, 0) <= X; ++i) {
+        const float wgt = wy * axis_weight(X, cols(i, 2), cols(i, 3), g.width, g.blend);
+        if (wgt <= 0.0F) continue;
+        const TileDesc& T = tiles[g.tile_base + j * g.nx + i];
+        const std::uint64_t ti = T.px_base + std::uint64_t(Y - T.y0) * T.w + (X - T.x0);
+        wall += wgt;
+        const float dv = tdisp[ti];
+        if (__builtin_isfinite(dv)) {
+          wsum += wgt;
+          dsum += wgt * dv;
+          if (conf != nullptr) csum += wgt * tconf[ti];
+          if (wgt > wbest) {
+            wbest = wgt;
+            dbest = dv;
+            if (lab != nullptr) lbest = tlab[ti];
+          }
         }
       }
     }
+  };
+  if (staged) {
+    blend([&](int i, int k) { return scols[4 * i + k]; }, [&](int j, int k) { return srows[4 * j + k]; });
+  } else {
+    const auto gc = gptr(g.cols), gr = gptr(g.rows);
+    blend([&](int i, int k) { return gc[4 * i + k]; }, [&](int j, int k) { return gr[4 * j + k]; });
   }
   float dv = kInf;
   float cv = 0.0F;
