@@ -329,8 +329,20 @@ __global__ __launch_bounds__(1024) void k_cc_border(const float* __restrict__ d,
   const int x = blockIdx.x * kCcW + tx, y = blockIdx.y * kCcH + ty;
   if (x >= w || y >= h) return;
   const int i = y * w + x;
-  if (tx == kCcW - 1 && x + 1 < w && fabsf(d[i + 1] - d[i]) <= max_diff) cc_union(p, i, i + 1);
-  if (ty == kCcH - 1 && y + 1 < h && fabsf(d[i + w] - d[i]) <= max_diff) cc_union(p, i, i + w);
+  auto joined = [&](int a, int b) { return fabsf(d[b] - d[a]) <= max_diff; };  // b right of or below a
+  // A border edge is skipped where the same edge one row up (right border)
+  // or one column left (bottom border) joins the same two blocks and the
+  // in-block edges between them are joined too: the partition already holds
+  // it.  The in-block edges are exact after the local phase and each chain of
+  // skips ends at the block's first row or column, so nothing is lost.
+  if (tx == kCcW - 1 && x + 1 < w && joined(i, i + 1)) {
+    const bool implied = ty > 0 && joined(i - w, i + 1 - w) && joined(i - w, i) && joined(i + 1 - w, i + 1);
+    if (!implied) cc_union(p, i, i + 1);
+  }
+  if (ty == kCcH - 1 && y + 1 < h && joined(i, i + w)) {
+    const bool implied = tx > 0 && joined(i - 1, i) && joined(i - 1 + w, i + w) && joined(i - 1, i - 1 + w);
+    if (!implied) cc_union(p, i, i + w);
+  }
 }
 
 // Component sizes, counted up to min_size: the lanes of a wave that share a
