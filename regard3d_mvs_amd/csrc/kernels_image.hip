@@ -223,10 +223,12 @@ __global__ void k_apply_mask(float* __restrict__ map, std::size_t n, const std::
 // |d_j - d_i| <= max_diff (the relation is symmetric, so the segments do not
 // depend on the visiting order).  Here: union-find in three steps, every
 // parent pointer only ever moving to a smaller pixel index (no cycles):
-//   1. each 32x32 block unions its pixels in LDS and points them to the
+//   1. each 32x32 block labels its horizontal runs with a wave ballot,
+//      unions the runs vertically in LDS and points its pixels to the
 //      block-local root;
 //   2. pixels on block borders union across them in HBM (atomicMin links,
-//      Playne & Hawick 2018, with path halving);
+//      Playne & Hawick 2018, with path halving), except where the
+//      neighbouring border edge already joins the same two runs;
 //   3. every pixel finds its root and counts it, saturating at min_size (only
 //      "fewer than min_size" matters, and big segments would serialise on one
 //      counter), then small segments are invalidated.
