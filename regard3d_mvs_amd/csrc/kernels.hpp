@@ -18,8 +18,6 @@ void launch_pyramid_u16(const std::uint16_t* in, std::int64_t is, int w, int h, 
                         std::uint16_t* out, std::int64_t os, hipStream_t s);
 void launch_pyramid_u8(const std::uint8_t* in, std::int64_t is, int w, int h, std::uint32_t* tmp, std::uint8_t* out,
                        std::int64_t os, bool threshold, hipStream_t s);
-void launch_mirror_u16(const std::uint16_t* in, std::int64_t is, int w, int h, std::uint16_t* out, std::int64_t os,
-                       hipStream_t s);
 void launch_mirror_u8(const std::uint8_t* in, std::int64_t is, int w, int h, std::uint8_t* out, std::int64_t os,
                       hipStream_t s);
 // rows[y] = {first, last} valid column (mask >= 128); mask == nullptr: {0, w - 1}.

@@ -870,10 +870,6 @@ void launch_combine(const CombinePair* pairs, const CombineGrid& g, float* depth
   k_combine<<<dim3((g.w + 255) / 256, g.h), 256, 0, s>>>(pairs, g, depth, ds, conf, cs, mask, ms);
 }
 
-void launch_mirror_u16(const std::uint16_t* in, std::int64_t is, int w, int h, std::uint16_t* out, std::int64_t os,
-                       hipStream_t s) {
-  k_mirror<std::uint16_t><<<dim3((w + 255) / 256, h), 256, 0, s>>>(in, is, w, h, out, os);
-}
 
 void launch_mirror_u8(const std::uint8_t* in, std::int64_t is, int w, int h, std::uint8_t* out, std::int64_t os,
                       hipStream_t s) {

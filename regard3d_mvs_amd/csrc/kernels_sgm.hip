@@ -304,7 +304,7 @@ __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, 
   const auto cr = gptr(J.cr) + std::size_t(Y) * J.cr_stride;
   const std::uint64_t* bc = bcorr + T.band_base + y;
   const int pw1 = J.pwidth - 1;
-  const bool mir = J.mirror_census != 0;
+  const bool mir = J.mirrored != 0;
   auto bcol = [&](int X) { return mir ? J.width - 1 - X : X; };  // column of cl
   auto pcol = [&](int c) { return mir ? pw1 - c : c; };         // column of cr
   // Partner columns a cost of this row can use: X - dmin - label for X in the
@@ -563,6 +563,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
   const std::uint64_t* tbc = bcorr + T.band_base;
   const std::uint16_t* img = J.img;
   const int img_stride = static_cast<int>(J.img_stride);
+  // Base columns of a mirrored job read the unmirrored image backwards.
+  const int img_x0 = J.mirrored != 0 ? J.width - 1 - T.x0 : T.x0, img_dx = J.mirrored != 0 ? -1 : 1;
   const std::uint8_t* Ct = C + T.cell_base;
   std::uint16_t* St = S + T.cell_base;
 
@@ -573,7 +575,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
     const int x = x0 + i * DX, y = y0 + i * DY;
     b.rec = ldb<std::uint64_t>(tpix, static_cast<std::uint32_t>(y * T.w + x) * 8U);
     b.rs = ldb<std::uint32_t>(tbc, static_cast<std::uint32_t>((x >> T.band_shift) * T.h + y) * 8U);
-    b.ip = ldb<std::uint16_t>(img, static_cast<std::uint32_t>((T.y0 + y) * img_stride + T.x0 + x) * 2U);
+    b.ip = ldb<std::uint16_t>(img, static_cast<std::uint32_t>((T.y0 + y) * img_stride + img_x0 + img_dx * x) * 2U);
   };
   // Unpacks a batch into the ring (slot l8 of the group); last_ip carries the
   // intensity of the batch's last step to the next batch's first.

@@ -207,13 +207,6 @@ DImg<std::uint8_t> pyramid_u8(Context& c, const DImg<std::uint8_t>& img, bool th
   return out;
 }
 
-DImg<std::uint16_t> mirror_u16(Context& c, const DImg<std::uint16_t>& img) {
-  DImg<std::uint16_t> out = alloc_img<std::uint16_t>(c, img.w, img.h);
-  launch_mirror_u16(img.p, img.s, img.w, img.h, out.p, out.s, c.stream);
-  check_launch();
-  return out;
-}
-
 DImg<std::uint8_t> mirror_u8(Context& c, const DImg<std::uint8_t>& img) {
   if (img.empty()) return DImg<std::uint8_t>{};
   DImg<std::uint8_t> out = alloc_img<std::uint8_t>(c, img.w, img.h);
@@ -281,7 +274,6 @@ struct View {
   std::deque<DImg<std::uint16_t>> pyr;  // level l at pyr[l - 1]
   std::deque<DImg<std::uint8_t>> mpyr;
   std::map<int, DImg<std::uint64_t>> cen;
-  std::map<int, DImg<std::uint16_t>> mir;
   std::map<int, DImg<std::uint8_t>> mir_mask;
 
   const DImg<std::uint16_t>& image(Context& c, int l) {
@@ -298,11 +290,6 @@ struct View {
     if (it == cen.end()) it = cen.emplace(l, census(c, image(c, l))).first;
     return it->second;
   }
-  const DImg<std::uint16_t>& mirrored(Context& c, int l) {
-    auto it = mir.find(l);
-    if (it == mir.end()) it = mir.emplace(l, mirror_u16(c, image(c, l))).first;
-    return it->second;
-  }
   const DImg<std::uint8_t>& mirrored_valid(Context& c, int l) {
     auto it = mir_mask.find(l);
     if (it == mir_mask.end()) it = mir_mask.emplace(l, mirror_u8(c, valid(c, l))).first;
@@ -314,7 +301,6 @@ struct View {
     for (const auto& x : pyr) b += 2 * x.pixels();
     for (const auto& x : mpyr) b += x.pixels();
     for (const auto& [l, x] : cen) b += 8 * x.pixels();
-    for (const auto& [l, x] : mir) b += 2 * x.pixels();
     for (const auto& [l, x] : mir_mask) b += x.pixels();
     return b;
   }
@@ -332,7 +318,7 @@ struct Job {
   int D = 0;
   bool confidence = true;
   bool labels = false;
-  bool mirror_census = false;  // cl / cr are the unmirrored images' Census (JobDesc)
+  bool mirrored = false;  // base / cl / cr are the unmirrored images (JobDesc::mirrored)
 };
 
 DBuf<std::int32_t> pack_rows(Context& c, const std::int32_t* base, const std::int32_t* partner, int h) {
@@ -434,7 +420,7 @@ Aggregation layout_jobs(Context& c, std::vector<Job>& jobs, const r3dg_options& 
     d.cl_stride = J.cl.s;
     d.cr = J.cr.p;
     d.cr_stride = J.cr.s;
-    d.mirror_census = J.mirror_census ? 1 : 0;
+    d.mirrored = J.mirrored ? 1 : 0;
     d.rows4 = J.rows4.get();
     d.lo = J.ranges != nullptr ? J.ranges->lo.p : nullptr;
     d.num = J.ranges != nullptr ? J.ranges->num.p : nullptr;
@@ -865,12 +851,13 @@ std::unique_ptr<LevelPrep> prepare_level(Context& c, View& Lv, View& Rv, int l, 
       P->rranges = ranges_from_coarse(c, *coarse_right, wr, h, *in.mR, P->lrows.get(), in.dmin, in.nd, o, 1, true);
     }
     Job b;
-    b.base = &Rv.mirrored(c, l);
-    b.partner = &Lv.mirrored(c, l);
-    // The unmirrored images' Census read backwards (JobDesc::mirror_census).
+    // The unmirrored images and their Census, read backwards
+    // (JobDesc::mirrored); the partner image only gives its width.
+    b.base = &Rv.image(c, l);
+    b.partner = &Lv.image(c, l);
     b.cl = view_of(Rv.census_at(c, l));
     b.cr = view_of(Lv.census_at(c, l));
-    b.mirror_census = true;
+    b.mirrored = true;
     b.lrows = P->mrows_base.get();
     b.rrows = P->mrows_partner.get();
     b.rows4 = pack_rows(c, P->mrows_base.get(), P->mrows_partner.get(), h);

@@ -55,12 +55,13 @@ struct JobDesc {
   std::int32_t p1;
   std::int32_t invalid_cost;
   std::int32_t confidence;   // select writes confidence
-  // 1: cl / cr hold the Census of the unmirrored images of a mirrored job;
-  // base column X reads cl[width - 1 - X], partner column c reads
-  // cr[pwidth - 1 - c].  The Census of a mirrored image is that of the image
-  // read backwards with its bits permuted (neighbour (dy, dx) <-> (dy, -dx)),
-  // and a permutation applied to both sides leaves Hamming distances unchanged.
-  std::int32_t mirror_census;
+  // 1: a mirrored job whose img, cl and cr are those of the unmirrored
+  // images: base column X reads img[width - 1 - X] and cl[width - 1 - X],
+  // partner column c reads cr[pwidth - 1 - c].  The Census of a mirrored
+  // image is that of the image read backwards with its bits permuted
+  // (neighbour (dy, dx) <-> (dy, -dx)), and a permutation applied to both
+  // sides leaves Hamming distances unchanged.
+  std::int32_t mirrored;
   std::uint16_t lut[256];    // P2 by 8-bit intensity step (stereo_matcher.cpp:69-79)
 };
 
