@@ -83,10 +83,10 @@ __device__ __forceinline__ void label_range(const JobDesc& J, int X, int Y, int&
 }
 
 // ------------------------------------------------------------------ layout
-// Inclusive scan of v over a block of 256 threads (4 waves): shuffles inside
-// each wave, the wave totals through LDS.  Returns this thread's inclusive
-// prefix and the block total; every thread of the block must call it.
-template <typename U>
+// Inclusive scan of v over a block of W waves: shuffles inside each wave, the
+// wave totals through LDS (wsum[W]).  Returns this thread's inclusive prefix
+// and the block total; every thread of the block must call it.
+template <int W, typename U>
 __device__ __forceinline__ U block_scan_incl(U v, U* wsum, U& total) {
   const int lane = static_cast<int>(threadIdx.x & 63), wave = static_cast<int>(threadIdx.x >> 6);
 #pragma unroll
@@ -96,10 +96,13 @@ __device__ __forceinline__ U block_scan_incl(U v, U* wsum, U& total) {
   }
   if (lane == 63) wsum[wave] = v;
   __syncthreads();
-  U off = 0;
+  U off = 0, all = 0;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) off += k < wave ? wsum[k] : U(0);
-  total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  for (int k = 0; k < W; ++k) {
+    off += k < wave ? wsum[k] : U(0);
+    all += wsum[k];
+  }
+  total = all;
   __syncthreads();  // wsum is reused by the next call
   return v + off;
 }
@@ -184,7 +187,8 @@ __global__ __launch_bounds__(256) void k_tile_row_scan(const JobDesc* __restrict
 // Totals of one tile (tile_cells, tile_max) and the band corrections: an
 // exclusive scan of the band-row cell counts in band-major order, minus the
 // row offset of the band's first pixel.
-__global__ __launch_bounds__(256) void k_tile_cells(const TileDesc* __restrict__ tiles,
+constexpr int kCellsThreads = 1024;  // k_tile_cells: one block of 16 waves per tile
+__global__ __launch_bounds__(kCellsThreads) void k_tile_cells(const TileDesc* __restrict__ tiles,
                                                     const std::uint64_t* __restrict__ row_cells,
                                                     const std::uint32_t* __restrict__ row_max,
                                                     const std::uint64_t* __restrict__ bstart,
@@ -193,14 +197,14 @@ __global__ __launch_bounds__(256) void k_tile_cells(const TileDesc* __restrict__
                                                     std::uint32_t* __restrict__ tile_max) {
   constexpr int kItems = 4;  // consecutive band rows per thread and pass
   const TileDesc T = tiles[blockIdx.x];
-  __shared__ std::uint64_t wsum[4];
+  __shared__ std::uint64_t wsum[kCellsThreads / 64];
   __shared__ std::uint32_t smax;
   if (threadIdx.x == 0) smax = 0;
   __syncthreads();
-  for (int y = threadIdx.x; y < T.h; y += 256) atomicMax(&smax, row_max[T.row_base + y]);
+  for (int y = threadIdx.x; y < T.h; y += kCellsThreads) atomicMax(&smax, row_max[T.row_base + y]);
   const int n = T.nbands * T.h;
   std::uint64_t carry = 0;
-  for (int base = 0; base < n; base += 256 * kItems) {
+  for (int base = 0; base < n; base += kCellsThreads * kItems) {
     const int e0 = base + kItems * static_cast<int>(threadIdx.x);
     std::uint64_t v[kItems], b0[kItems], sum = 0;
 #pragma unroll
@@ -217,7 +221,7 @@ __global__ __launch_bounds__(256) void k_tile_cells(const TileDesc* __restrict__
       sum += v[i];
     }
     std::uint64_t total;
-    std::uint64_t before = carry + block_scan_incl(sum, wsum, total) - sum;
+    std::uint64_t before = carry + block_scan_incl<kCellsThreads / 64>(sum, wsum, total) - sum;
 #pragma unroll
     for (int i = 0; i < kItems; ++i) {
       if (e0 + i < n) bcorr[T.band_base + e0 + i] = before - b0[i];
@@ -1248,7 +1252,7 @@ void launch_tile_row_scan(const JobDesc* jobs, const TileDesc* tiles, const std:
 void launch_tile_cells(const TileDesc* tiles, int ntiles, const std::uint64_t* row_cells, const std::uint32_t* row_max,
                        const std::uint64_t* bstart, std::uint64_t* bcorr, std::uint64_t* tile_cells,
                        std::uint32_t* tile_max, hipStream_t s) {
-  k_tile_cells<<<dim3(ntiles), 256, 0, s>>>(tiles, row_cells, row_max, bstart, bcorr, tile_cells, tile_max);
+  k_tile_cells<<<dim3(ntiles), kCellsThreads, 0, s>>>(tiles, row_cells, row_max, bstart, bcorr, tile_cells, tile_max);
 }
 
 void launch_cost(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t* row_prefix, int ntiles,
