@@ -55,18 +55,95 @@ def measured_traffic(config):
     return None
 
 
-def cpu_baseline(pair, options, rows, threads):
-    """The oracle on rows [0, rows) of the pair; rate scaled to whole pairs."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads(args) -> int:
+    """All host cores of this process's CPU share (at most 16: the share of
+    one GPU on the GPU hosts)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return args.cpu_threads or min(16, n)
+
+
+def best_of(runs, fn):
+    best, res = float("inf"), None
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        res = fn()
+        best = min(best, time.perf_counter() - t0)
+    return best, res
+
+
+def cpu_baseline(pair, options, cells_per_pair, args):
+    """The oracle (the CPU restatement of the reference matcher) timed on this
+    host, best of `cpu_runs`: on all cores over a band of the pair's rows, and
+    on one core over a square crop; both keep the pair's level structure, and
+    rates are scaled to whole pairs by aggregated cells."""
     from oracle import oracle as O
-    h = pair.left.shape[0]
-    rows = min(rows, h)
-    t0 = time.perf_counter()
-    r = O.match_stereo(pair.left[:rows], pair.right[:rows], None, None, pair.dmin, pair.nd, options, threads=threads)
-    dt = time.perf_counter() - t0
-    frac = rows / h
-    return {"value": frac / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
-            "sample": f"rows 0..{rows - 1} of the {pair.left.shape[1]}x{h} pair ({frac:.3f} of it, "
-                      f"{r.levels + 1} levels, {r.cells} cells) in {dt:.2f} s, scaled to whole pairs"}
+    h, w = pair.left.shape
+    threads = cpu_threads(args)
+    rows = min(h, args.cpu_rows)
+    t_all, r_all = best_of(args.cpu_runs, lambda: O.match_stereo(
+        pair.left[:rows], pair.right[:rows], None, None, pair.dmin, pair.nd, options, threads=threads))
+    side = min(h, w, args.cpu_crop)
+    cl = np.ascontiguousarray(pair.left[:side, :side])
+    cr = np.ascontiguousarray(pair.right[:side, :side])
+    t_one, r_one = best_of(args.cpu_runs, lambda: O.match_stereo(cl, cr, None, None, pair.dmin, pair.nd, options,
+                                                                  threads=1))
+    v_all = r_all.cells / t_all / cells_per_pair
+    v_one = r_one.cells / t_one / cells_per_pair
+    log(f"bench: CPU baseline {v_all:.4f} pairs/s on {threads} threads, {v_one:.5f} on 1")
+    return {"value": v_all, "unit": "pairs/s", "cores": threads, "kind": "port",
+            "sample": f"rows 0..{rows - 1} of the {w}x{h} pair ({r_all.levels + 1} levels, {r_all.cells} cells) on "
+                      f"{threads} threads, best of {args.cpu_runs}: {t_all:.2f} s; scaled to whole pairs by cells "
+                      f"({cells_per_pair} per pair)",
+            "one_core": {"value": v_one, "unit": "pairs/s", "cores": 1,
+                         "sample": f"the {side}x{side} top-left crop ({r_one.levels + 1} levels, {r_one.cells} cells), "
+                                   f"best of {args.cpu_runs}: {t_one:.2f} s; scaled by cells"},
+            "cpu_model": cpu_model(),
+            "host_cpus_visible": os.cpu_count()}
+
+
+# The reference's own published CPU figures for its bench_stereo default
+# (1024x768, 128 disparities; bench/README.md:23-30, clang-cl build, 16-core
+# desktop): Mcell/s of full-range cells.
+PUBLISHED_MCELLS = {"full_1_thread": 150.0, "full_32_threads": 363.0, "hier_1_thread": 277.0,
+                    "hier_32_threads": 515.0}
+
+
+def calibration(m, args):
+    """bench_stereo's default case (1024x768x128, full range and automatic
+    levels) through the oracle on 1 and all threads and through the GPU
+    matcher, in Mcell/s of full-range cells (w * h * D / s), beside the
+    reference's published figures."""
+    from oracle import oracle as O
+    from regard3d_mvs_amd import StereoOptions, synth
+    w, h, nd = 1024, 768, 128
+    left, right = synth.bench_stereo_pair(w, h, nd)
+    full = w * h * nd
+    threads = cpu_threads(args)
+    out = {"input": "bench_stereo.cpp default: 1024x768, 128 disparities, multi-scale noise, smooth shift",
+           "unit": "Mcell/s (full-range cells w*h*D per second)", "threads": threads,
+           "published_reference_clang_cl": PUBLISHED_MCELLS}
+    for name, levels in (("full", 0), ("hier", -1)):
+        o = StereoOptions(levels=levels)
+        t1, r1 = best_of(2, lambda: O.match_stereo(left, right, None, None, 0, nd, o, threads=1))
+        tn, _ = best_of(3, lambda: O.match_stereo(left, right, None, None, 0, nd, o, threads=threads))
+        m.match(left, right, None, None, 0, nd, o)
+        tg, g = best_of(5, lambda: m.match(left, right, None, None, 0, nd, o))
+        exact = bool(np.array_equal(g.disparity.view(np.uint32), r1.disparity.view(np.uint32)))
+        out[name] = {"levels": r1.levels, "cells_per_pixel": r1.cells / (w * h),
+                     "oracle_1_thread": full / t1 / 1e6, f"oracle_{threads}_threads": full / tn / 1e6,
+                     "gpu_host_io": full / tg / 1e6, "gpu_bit_exact": exact}
+        log(f"bench: calibration {name}: {json.dumps(out[name])}")
+    return out
 
 
 def _same(a, b):
@@ -192,10 +269,9 @@ def bench_batch(args) -> int:
     from regard3d_mvs_amd.native import Matcher
 
     t = time.perf_counter()
-    src = workloads.source_pair()
-    reqs = workloads.scene(src, views=args.views)
-    log(f"bench: source {workloads.SOURCE_W}x{workloads.SOURCE_H} and {len(reqs)} pairs in "
-        f"{time.perf_counter() - t:.1f} s; {sum(r.left.size for r in reqs) / len(reqs) / 1e6:.1f} MP mean")
+    reqs = workloads.scene(views=args.views)
+    log(f"bench: {len(reqs)} pairs generated in {time.perf_counter() - t:.1f} s; "
+        f"{sum(r.left8.size for r in reqs) / len(reqs) / 1e6:.1f} MP mean")
     o = StereoOptions()
     dev = torch.device("cuda:0")
 
@@ -263,8 +339,7 @@ def bench_serve(args) -> int:
     from regard3d_mvs_amd import StereoOptions, workloads
     from regard3d_mvs_amd.native import Matcher
 
-    src = workloads.source_pair()
-    reqs = workloads.stream(src, args.requests)
+    reqs = workloads.stream(args.requests)
     log(f"bench: {len(reqs)} requests, {sum(r.left.size for r in reqs) / len(reqs) / 1e6:.2f} MP mean, "
         f"{len(set(r.view for r in reqs))} views")
     o = StereoOptions()
@@ -336,8 +411,12 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=2, choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-rows", type=int, default=4032, help="rows of the CPU-baseline band (0: skip)")
+    ap.add_argument("--cpu-rows", type=int, default=1536, help="rows of the all-core CPU-baseline band")
+    ap.add_argument("--cpu-crop", type=int, default=1536, help="side of the one-core CPU-baseline crop")
+    ap.add_argument("--cpu-runs", type=int, default=3, help="best-of runs of the CPU baseline (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-calibrate", dest="calibrate", action="store_false",
+                    help="skip the 1024x768x128 calibration against the reference's published CPU figures")
     ap.add_argument("--workload", default="pair", choices=["pair", "batch", "serve", "copy", "rectify", "combine"],
                     help="pair: one pair per call (default); batch: the 160-pair scene, batched vs plain; "
                          "serve: a Poisson request stream through the serving loop vs plain; copy: one 1 GiB "
@@ -372,10 +451,17 @@ def main() -> int:
     if args.workload == "combine":
         return bench_combine(args)
 
+    return bench_pair(args)
+
+
+def bench_pair(args) -> int:
+    """One pair per call (the plain form).  value: host uint8 images in, host
+    float disparity and confidence out, per wall second; the same pair with
+    images already in HBM and maps left there is reported beside it."""
     import torch
 
     from regard3d_mvs_amd import StereoOptions, synth
-    from regard3d_mvs_amd.native import Matcher
+    from regard3d_mvs_amd.native import Matcher, host_empty
 
     t = time.perf_counter()
     pair = synth.config_pair(args.config)
@@ -383,17 +469,19 @@ def main() -> int:
     log(f"bench: generated {WORKLOADS[args.config]} in {time.perf_counter() - t:.1f} s")
     options = StereoOptions(levels=0) if args.config == 0 else StereoOptions()
 
-    dev = torch.device("cuda:0")
-    left = torch.from_numpy(pair.left.view(np.int16)).to(dev).view(torch.uint16)
-    right = torch.from_numpy(pair.right.view(np.int16)).to(dev).view(torch.uint16)
-    disp = torch.empty((h, w), dtype=torch.float32, device=dev)
-    conf = torch.empty((h, w), dtype=torch.float32, device=dev)
-    torch.cuda.synchronize()
+    # Host buffers: the uint8 images (u16 = u8 * 257 is formed on the device)
+    # and the float maps, in page-locked memory of the library's runtime.
+    left8, k1 = host_empty((h, w), np.uint8)
+    right8, k2 = host_empty((h, pair.right.shape[1]), np.uint8)
+    left8[...] = (pair.left // 257).astype(np.uint8)
+    right8[...] = (pair.right // 257).astype(np.uint8)
+    hdisp, k3 = host_empty((h, w), np.float32)
+    hconf, k4 = host_empty((h, w), np.float32)
 
     m = Matcher(0)
     m.set_profiling(True)
     for i in range(args.warmup):
-        r = m.match(left, right, None, None, pair.dmin, pair.nd, options, out=(disp, conf))
+        r = m.match(left8, right8, None, None, pair.dmin, pair.nd, options, out=(hdisp, hconf))
         log(f"bench: warmup {i + 1}/{args.warmup}: {r.seconds * 1e3:.1f} ms")
     torch.cuda.synchronize()
     lat, agg_s, agg_b, launches = [], 0.0, 0, 0
@@ -401,7 +489,7 @@ def main() -> int:
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s0 = time.perf_counter()
-        r = m.match(left, right, None, None, pair.dmin, pair.nd, options, out=(disp, conf))
+        r = m.match(left8, right8, None, None, pair.dmin, pair.nd, options, out=(hdisp, hconf))
         lat.append(time.perf_counter() - s0)
         agg_s += r.stage_seconds["aggregate"]
         agg_b += r.aggregate_bytes
@@ -412,29 +500,57 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     steps = args.steps
     pairs_s = steps / elapsed
-    cells = r.cells
-    valid = float(torch.isfinite(disp).float().mean().item())
+    cells, tile_px = r.cells, r.tile_pixels
+    valid = float(np.isfinite(hdisp).mean())
     log("bench: stage ms/pair: " + ", ".join(f"{k} {1e3 * v / steps:.2f}" for k, v in stages.items()))
-    log(f"bench: {steps} pairs in {elapsed:.3f} s; {cells} cells ({cells / (w * h):.1f}/px), "
-        f"{r.tile_pixels} tile px, valid {valid:.3f}")
+    log(f"bench: {steps} pairs host u8 in -> host f32 out in {elapsed:.3f} s; {cells} cells "
+        f"({cells / (w * h):.1f}/px), {tile_px} tile px, valid {valid:.3f}")
 
-    achieved = agg_b / agg_s if agg_s > 0 else 0.0
+    # The same pair with the images resident in HBM and the maps left there.
+    dev = torch.device("cuda:0")
+    dleft = torch.from_numpy(pair.left.view(np.int16)).to(dev).view(torch.uint16)
+    dright = torch.from_numpy(pair.right.view(np.int16)).to(dev).view(torch.uint16)
+    ddisp = torch.empty((h, w), dtype=torch.float32, device=dev)
+    dconf = torch.empty((h, w), dtype=torch.float32, device=dev)
+    m.set_profiling(False)
+    m.match(dleft, dright, None, None, pair.dmin, pair.nd, options, out=(ddisp, dconf))
+    torch.cuda.synchronize()
+    d0 = time.perf_counter()
+    for _ in range(steps):
+        m.match(dleft, dright, None, None, pair.dmin, pair.nd, options, out=(ddisp, dconf))
+    torch.cuda.synchronize()
+    dev_elapsed = time.perf_counter() - d0
+    same = bool(np.array_equal(ddisp.cpu().numpy().view(np.uint32), hdisp.view(np.uint32)) and
+                np.array_equal(dconf.cpu().numpy().view(np.uint32), hconf.view(np.uint32)))
+    log(f"bench: {steps} pairs resident in HBM in {dev_elapsed:.3f} s; maps equal to the host run: {same}")
+
+    # Roofline of the aggregation: the path scans' algorithmic bytes per
+    # matching direction and level, 30 C + 176 P (C cells, P tile pixels),
+    # over the event-timed wall time of the scans.
+    alg = 30 * cells + 176 * tile_px
+    per_launch = alg / max(r.aggregate_launches, 1)
+    achieved = alg * steps / agg_s if agg_s > 0 else 0.0
+    own = agg_b / agg_s if agg_s > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": measured_traffic(args.config),
-                "kernel": "k_path (8 SGM path scans per level)",
-                "algorithmic_bytes_per_launch": agg_b / max(launches, 1),
-                # The 8 scans of a level run on two streams (more on small
-                # levels), so launches overlap: this is the aggregation's wall
-                # time per launch, the union of the launches' intervals.
+                "kernel": "k_path (8 SGM path scans per level and chunk)",
+                "algorithmic_bytes": "30 C + 176 P per pair (C cells, P tile pixels, all levels and both directions)",
+                "algorithmic_bytes_per_launch": per_launch,
+                # The scans of a level overlap on two or more streams, so a
+                # launch's share is the aggregation's wall time per launch,
+                # the union of the launches' intervals.
                 "avg_launch_ms": 1e3 * agg_s / max(launches, 1),
-                "launch_timing": "aggregation wall time / launches (two streams)"}
+                "launch_timing": "HIP events around the path launches of each chunk on the library's stream "
+                                 "(which waits for the side streams); wall time / launches",
+                "own_model": {"bytes_per_launch": agg_b / max(launches, 1), "achieved": own / 1e9,
+                              "frac": own / HBM_PEAK,
+                              "formula": "(40 - 2 w) C + 80 P: C read by 8 scans, S written by 8 and read by "
+                                         "8 - w (w streams), record and intensity per pixel and scan"}}
 
     cpu = None
-    if args.cpu_rows > 0:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"bench: CPU baseline on {args.cpu_rows} rows with {threads} threads ...")
-        cpu = cpu_baseline(pair, options, args.cpu_rows, threads)
-        log(f"bench: CPU baseline {cpu['value']:.4f} pairs/s ({cpu['sample']})")
+    if args.cpu_runs > 0:
+        cpu = cpu_baseline(pair, options, cells, args)
+    calib = calibration(m, args) if args.calibrate else None
 
     out = {
         "metric": METRIC,
@@ -451,14 +567,20 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": WORKLOADS[args.config], "width": w, "height": h, "disparity_min": pair.dmin,
                    "num_disparities": pair.nd, "levels": r.levels, "paths": 8, "form": "plain, one pair per call",
-                   "inputs": "uint16 (u8 x 257) resident in HBM", "seed": pair.seed},
+                   "inputs": "host uint8 images in page-locked memory, widened to u8 * 257 on the device; "
+                             "host float disparity and confidence out", "seed": pair.seed},
         "gcell_path_per_s": cells * 8 * steps / elapsed / 1e9,
         "cells_per_pair": cells,
+        "tile_pixels_per_pair": tile_px,
         "latency_ms": {"p50": 1e3 * percentile(lat, 50), "p95": 1e3 * percentile(lat, 95),
                        "p99": 1e3 * percentile(lat, 99)},
+        "device_resident": {"pairs_per_s": steps / dev_elapsed, "ms_per_pair": 1e3 * dev_elapsed / steps,
+                            "gcell_path_per_s": cells * 8 * steps / dev_elapsed / 1e9,
+                            "maps_equal_host_run": same},
         "stage_ms_per_pair": {k: 1e3 * v / steps for k, v in stages.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "calibration": calib,
     }
     print(json.dumps(out), flush=True)
     m.close()

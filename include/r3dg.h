@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define R3DG_ABI_VERSION 2
+#define R3DG_ABI_VERSION 3
 
 typedef enum r3dg_status {
   R3DG_OK = 0,
@@ -44,15 +44,26 @@ typedef enum r3dg_status {
 
 typedef enum r3dg_memory { R3DG_HOST = 0, R3DG_DEVICE = 1 } r3dg_memory;
 
+/* Element format of an image view.  R3DG_FORMAT_U8X257 marks uint8 data given
+ * where a uint16 image is expected (left, right and view images, Census and
+ * cost-volume inputs): the bytes are transferred as they are and widened on
+ * the device to u8 * 257, the reference's 8-to-16-bit convention
+ * (io/image_io.hpp:25-26); stride is then in bytes. */
+typedef enum r3dg_format { R3DG_FORMAT_NATIVE = 0, R3DG_FORMAT_U8X257 = 1 } r3dg_format;
+
 /* A strided single-channel image; stride in elements (>= width).  A view with
- * data == NULL or width == 0 or height == 0 is empty (masks: all valid). */
+ * data == NULL or width == 0 or height == 0 is empty (masks: all valid).
+ * Host memory from r3dg_host_alloc is transferred by DMA directly; other
+ * host memory is staged through the context's pinned blocks.  Device memory
+ * must hold its final contents when the call starts: the library's stream
+ * does not wait for work the caller queued on other streams. */
 typedef struct r3dg_image {
   void* data;
   int32_t width;
   int32_t height;
   int64_t stride;
   int32_t memory; /* r3dg_memory */
-  int32_t reserved;
+  int32_t format; /* r3dg_format */
 } r3dg_image;
 
 typedef enum r3dg_subpixel {
@@ -154,6 +165,12 @@ void r3dg_default_options(r3dg_options* options);
 int32_t r3dg_ctx_create(int32_t device, uint64_t hbm_budget, r3dg_ctx** out);
 int32_t r3dg_ctx_destroy(r3dg_ctx* ctx);
 
+/* Page-locked host memory from the library's HIP runtime (no reference
+ * counterpart): images in it move between host and device by DMA with no
+ * staging copy.  r3dg_host_free releases it. */
+int32_t r3dg_host_alloc(size_t bytes, void** out);
+int32_t r3dg_host_free(void* p);
+
 /* Copies the message of the context's last failure into buf (NUL-terminated)
  * and returns the status of that failure. */
 int32_t r3dg_last_error(const r3dg_ctx* ctx, char* buf, size_t size);
@@ -164,8 +181,16 @@ int32_t r3dg_global_error(char* buf, size_t size);
 int32_t r3dg_set_profiling(r3dg_ctx* ctx, int32_t enabled);
 
 /* Cancellation flag polled between pyramid levels and tile stages; replaces
- * CancellationToken polling in parallel_for (core/thread_pool.cpp:41-70). */
+ * CancellationToken polling in parallel_for (core/thread_pool.cpp:41-70).
+ * The flag stays set until it is cleared (requested = 0), as a token stays
+ * requested until reset (core/cancel.hpp:22-24). */
 int32_t r3dg_cancel(r3dg_ctx* ctx, int32_t requested);
+
+/* A caller's cancellation test, polled at the same points as the flag: a
+ * non-zero poll(user) cancels the running call (R3DG_ERROR_CANCELLED).  NULL
+ * removes it.  Replaces current_cancellation() (core/cancel.hpp:39-44): the
+ * C++ layer passes the calling thread's CancellationToken. */
+int32_t r3dg_set_cancel_poll(r3dg_ctx* ctx, int32_t (*poll)(void* user), void* user);
 
 /* ---- matcher stages ------------------------------------------------------ */
 
@@ -250,6 +275,65 @@ int32_t r3dg_ranges_from_coarse(r3dg_ctx* ctx, r3dg_image coarse, int32_t width,
 
 /* detail::automatic_levels (stereo_matcher.hpp:128). Host-only arithmetic. */
 int32_t r3dg_automatic_levels(int32_t width, int32_t height, int32_t num_disparities);
+
+/* ---- row kernels (for callers that drive the row loop themselves) --------
+ * The reference's row-kernel interface (stereo/census.hpp:25-29,
+ * stereo/sgm_kernels.hpp:16-97) on the GPU: every pointer is host memory
+ * laid out as the reference describes; the call uploads what the kernel
+ * reads, runs it on the context's stream and writes the reference's outputs
+ * back (only the elements the reference writes). */
+
+/* census_row: rows[j] points to row y - 3 + j of an image padded by 4
+ * replicated pixels on both sides, at the padded start (n + 8 elements);
+ * writes out[0 .. n). */
+int32_t r3dg_census_row(r3dg_ctx* ctx, const uint16_t* const rows[7], uint64_t* out, size_t n);
+
+/* sgm::RowLabels (sgm_kernels.hpp:28-39): pixel x has labels lo[x] ..
+ * lo[x] + num[x] - 1 (num a positive multiple of 16), packed from offset[x]
+ * (offset[width] = total); path buffers pad every pixel by 8 on both sides. */
+typedef struct r3dg_row_labels {
+  const int32_t* lo;
+  const uint16_t* num;
+  const size_t* offset; /* [width + 1] */
+} r3dg_row_labels;
+
+/* sgm::census_costs (sgm_kernels.hpp:41-44): cost[offset[x] + j] =
+ * popcount(left[x] ^ rev[width - 1 - x + lo[x] + j - rev_origin]). */
+int32_t r3dg_census_costs(r3dg_ctx* ctx, const uint64_t* left, const uint64_t* rev, size_t width,
+                          const r3dg_row_labels* labels, int32_t rev_origin, uint16_t* cost);
+
+/* sgm::SgmRowArgs (sgm_kernels.hpp:54-73), same fields and meaning.  scratch:
+ * only its first buffer (the zero start of every path, path_stride(max_num)
+ * elements) is read; the reference's other scratch buffers are its own
+ * working storage and are not written back. */
+typedef struct r3dg_sgm_row_args {
+  const uint16_t* cost;
+  const uint16_t* p2;       /* [4][width] */
+  const uint16_t* prev;     /* [3][prev_labels path size] */
+  const uint16_t* prev_min; /* [3][width] */
+  uint16_t* cur;            /* [3][labels path size] */
+  uint16_t* cur_min;        /* [3][width] */
+  uint16_t* scratch;
+  uint16_t* sum;
+  r3dg_row_labels labels;
+  r3dg_row_labels prev_labels;
+  size_t width;
+  size_t max_num;
+  uint16_t p1;
+  int32_t direction;
+  int32_t has_prev;
+  int32_t accumulate;
+} r3dg_sgm_row_args;
+
+/* sgm::sgm_row (sgm_kernels.hpp:46-76): one row of one 4-path SGM pass. */
+int32_t r3dg_sgm_row(r3dg_ctx* ctx, const r3dg_sgm_row_args* args);
+
+/* sgm::wta_row (sgm_kernels.hpp:78-88): best (absolute label), best cost and
+ * second best per pixel; right_cost / right_best (may be NULL) are updated for
+ * every i = x - lo[x] - k + right_shift as the reference does. */
+int32_t r3dg_wta_row(r3dg_ctx* ctx, const uint16_t* sum, size_t width, const r3dg_row_labels* labels, uint16_t* best,
+                     uint16_t* best_cost, uint16_t* second, uint16_t* right_cost, uint16_t* right_best,
+                     ptrdiff_t right_shift);
 
 /* ---- rectification (the step before the matcher) ------------------------- */
 

@@ -162,6 +162,32 @@ int oracle_median3x3(const float* d, int w, int h, float* out) {
   });
 }
 
+// ext: per tile x0, y0, w, h, core c0, c1, r0, r1; d[i] / c[i]: tile i's maps
+// (c may be null: no confidence).
+int oracle_blend(int width, int height, int blend, int ntiles, const int* ext, const float* const* d,
+                 const float* const* c, float* out_d, float* out_c) {
+  return guarded([&] {
+    std::vector<oracle::BlendTile> tiles(static_cast<std::size_t>(ntiles));
+    for (int i = 0; i < ntiles; ++i) {
+      const int* e = ext + 8 * i;
+      tiles[std::size_t(i)] = oracle::BlendTile{e[0], e[1], e[2], e[3], {e[4], e[5], e[6], e[7]}, d[i],
+                                                c != nullptr ? c[i] : nullptr, nullptr};
+    }
+    oracle::ImgF disp, conf;
+    oracle::blend_tiles(width, height, blend, tiles, disp, c != nullptr ? &conf : nullptr, nullptr);
+    std::memcpy(out_d, disp.px.data(), sizeof(float) * disp.px.size());
+    if (c != nullptr) std::memcpy(out_c, conf.px.data(), sizeof(float) * conf.px.size());
+  });
+}
+
+int oracle_lr_mismatch(const float* map, int w, int h, const float* other, int wo, int sign, int max_diff,
+                       std::uint8_t* out) {
+  return guarded([&] {
+    const auto m = oracle::left_right_mismatch(copy_in(map, w, h), copy_in(other, wo, h), sign, max_diff);
+    std::memcpy(out, m.px.data(), m.px.size());
+  });
+}
+
 int oracle_match(const std::uint16_t* left, int wl, int h, const std::uint16_t* right, int wr,
                  const std::uint8_t* left_valid, const std::uint8_t* right_valid, int dmin, int nd,
                  const r3dg_options* o, int threads, float* disparity, float* confidence, std::int16_t* labels,

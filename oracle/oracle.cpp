@@ -684,17 +684,33 @@ static TilesOut match_tiles(const Ctx& ctx, int threads) {
 
   TilesOut out;
   out.tiles = static_cast<int>(laid.size());
-  std::vector<std::vector<float>> colw(laid.size());
+  std::vector<BlendTile> bt(laid.size());
   for (std::size_t i = 0; i < laid.size(); ++i) {
     out.cells += res[i].cells;
     out.tile_pixels += std::uint64_t(laid[i].t.w) * std::uint64_t(laid[i].t.h);
-    colw[i].resize(std::size_t(laid[i].t.w));
-    for (int x = 0; x < laid[i].t.w; ++x)
-      colw[i][std::size_t(x)] = axis_weight(laid[i].t.x0 + x, laid[i].core[0], laid[i].core[1], width, o.tile_blend);
+    bt[i] = BlendTile{laid[i].t.x0, laid[i].t.y0, laid[i].t.w, laid[i].t.h,
+                      {laid[i].core[0], laid[i].core[1], laid[i].core[2], laid[i].core[3]}, res[i].d.data(),
+                      ctx.confidence ? res[i].c.data() : nullptr, ctx.want_labels ? res[i].lab.data() : nullptr};
   }
-  out.disparity = ImgF(width, height, kInvalid);
-  if (ctx.confidence) out.confidence = ImgF(width, height, 0.0F);
-  if (ctx.want_labels) out.labels = Img<std::int16_t>(width, height, INT16_MIN);
+  blend_tiles(width, height, o.tile_blend, bt, out.disparity, ctx.confidence ? &out.confidence : nullptr,
+              ctx.want_labels ? &out.labels : nullptr);
+  return out;
+}
+
+// matcher:688-766: per row, the tiles in tile order; weighted mean where the
+// valid weight is at least half of the total, the dominant tile's value where
+// the mean is more than 1 away from it.
+void blend_tiles(int width, int height, int blend, const std::vector<BlendTile>& tiles, ImgF& disparity,
+                 ImgF* confidence, Img<std::int16_t>* labels) {
+  std::vector<std::vector<float>> colw(tiles.size());
+  for (std::size_t i = 0; i < tiles.size(); ++i) {
+    colw[i].resize(std::size_t(tiles[i].w));
+    for (int x = 0; x < tiles[i].w; ++x)
+      colw[i][std::size_t(x)] = axis_weight(tiles[i].x0 + x, tiles[i].core[0], tiles[i].core[1], width, blend);
+  }
+  disparity = ImgF(width, height, kInvalid);
+  if (confidence != nullptr) *confidence = ImgF(width, height, 0.0F);
+  if (labels != nullptr) *labels = Img<std::int16_t>(width, height, INT16_MIN);
   const auto W = std::size_t(width);
   std::vector<float> wsum(W), wall(W), dsum(W), csum(W), wbest(W), dbest(W);
   std::vector<std::int16_t> lbest(W);
@@ -706,11 +722,10 @@ static TilesOut match_tiles(const Ctx& ctx, int threads) {
     std::fill(wbest.begin(), wbest.end(), 0.0F);
     std::fill(dbest.begin(), dbest.end(), kInvalid);
     std::fill(lbest.begin(), lbest.end(), std::int16_t{INT16_MIN});
-    for (std::size_t i = 0; i < laid.size(); ++i) {
-      const TileResult& r = res[i];
-      const Tile& t = r.t;
+    for (std::size_t i = 0; i < tiles.size(); ++i) {
+      const BlendTile& t = tiles[i];
       if (Y < t.y0 || Y >= t.y0 + t.h) continue;
-      const float wy = axis_weight(Y, laid[i].core[2], laid[i].core[3], height, o.tile_blend);
+      const float wy = axis_weight(Y, t.core[2], t.core[3], height, blend);
       if (wy <= 0.0F) continue;
       const std::size_t row = std::size_t(Y - t.y0) * std::size_t(t.w);
       for (int x = 0; x < t.w; ++x) {
@@ -719,15 +734,15 @@ static TilesOut match_tiles(const Ctx& ctx, int threads) {
         const std::size_t ti = row + std::size_t(x);
         const auto pi = std::size_t(t.x0 + x);
         wall[pi] += wgt;
-        const float dv = r.d[ti];
+        const float dv = t.d[ti];
         if (std::isfinite(dv)) {
           wsum[pi] += wgt;
           dsum[pi] += wgt * dv;
-          if (ctx.confidence) csum[pi] += wgt * r.c[ti];
+          if (confidence != nullptr) csum[pi] += wgt * t.c[ti];
           if (wgt > wbest[pi]) {
             wbest[pi] = wgt;
             dbest[pi] = dv;
-            if (ctx.want_labels) lbest[pi] = r.lab[ti];
+            if (labels != nullptr) lbest[pi] = t.lab[ti];
           }
         }
       }
@@ -737,13 +752,12 @@ static TilesOut match_tiles(const Ctx& ctx, int threads) {
       if (wsum[x] > 0.0F && wsum[x] >= 0.5F * wall[x]) {
         dv = dsum[x] / wsum[x];
         if (std::abs(dv - dbest[x]) > 1.0F) dv = dbest[x];
-        if (ctx.confidence) out.confidence(int(x), Y) = csum[x] / wsum[x];
-        if (ctx.want_labels) out.labels(int(x), Y) = lbest[x];
+        if (confidence != nullptr) (*confidence)(int(x), Y) = csum[x] / wsum[x];
+        if (labels != nullptr) (*labels)(int(x), Y) = lbest[x];
       }
-      out.disparity(int(x), Y) = dv;
+      disparity(int(x), Y) = dv;
     }
   }
-  return out;
 }
 
 // matcher:777-805
@@ -752,6 +766,15 @@ static bool left_right_differ(const ImgF& map, const ImgF& other, int x, int y, 
   const auto xo = static_cast<int>(std::lround(static_cast<float>(x) + static_cast<float>(sign) * d));
   const float dother = xo >= 0 && xo < other.w ? other(xo, y) : kInvalid;
   return !std::isfinite(dother) || std::abs(d - dother) > static_cast<float>(max_diff);
+}
+
+// matcher:797-805 over a whole map: 1 where a finite value fails the check.
+ImgU8 left_right_mismatch(const ImgF& map, const ImgF& other, int sign, int max_diff) {
+  ImgU8 out(map.w, map.h, 0);
+  for (int y = 0; y < map.h; ++y)
+    for (int x = 0; x < map.w; ++x)
+      if (std::isfinite(map(x, y)) && left_right_differ(map, other, x, y, sign, max_diff)) out(x, y) = 1;
+  return out;
 }
 
 // matcher:919-970: BFS over 4-neighbours with |d_j - d_i| <= max_diff.

@@ -133,6 +133,18 @@ Ranges ranges_from_coarse(const ImgF& coarse, int width, int height, const ImgU8
                           const ImgU8& partner_valid, int partner_width, int partner_sign, int dmin, int nd,
                           const r3dg_options& o, bool mirrored);
 
+// A tile of blend_tiles: extent, core [c0, c1) x [r0, r1), and its maps
+// (w x h, row-major; c and lab may be null).
+struct BlendTile {
+  int x0, y0, w, h;
+  int core[4];
+  const float* d;
+  const float* c;
+  const std::int16_t* lab;
+};
+void blend_tiles(int width, int height, int blend, const std::vector<BlendTile>& tiles, ImgF& disparity,
+                 ImgF* confidence, Img<std::int16_t>* labels);
+ImgU8 left_right_mismatch(const ImgF& map, const ImgF& other, int sign, int max_diff);
 void remove_speckles(ImgF& d, int min_size, float max_diff);
 ImgF median3x3(const ImgF& d);
 

@@ -200,6 +200,36 @@ def median_filter_3x3(d):
     return out
 
 
+def blend(width, height, blend_px, tiles, confidence=True):
+    """blend_tiles: `tiles` is a list of (x0, y0, core (c0, c1, r0, r1),
+    disparity, confidence) with float32 maps of the tile's size."""
+    n = len(tiles)
+    ext = np.zeros((max(n, 1), 8), np.int32)
+    ds, cs = [], []
+    for i, (x0, y0, core, d, c) in enumerate(tiles):
+        d = _c(d, np.float32)
+        ext[i] = (x0, y0, d.shape[1], d.shape[0], *core)
+        ds.append(d)
+        cs.append(_c(c, np.float32) if confidence else None)
+    dp = (ctypes.c_void_p * max(n, 1))(*[a.ctypes.data for a in ds])
+    cp = (ctypes.c_void_p * max(n, 1))(*[a.ctypes.data for a in cs]) if confidence else None
+    out_d = np.empty((height, width), np.float32)
+    out_c = np.empty((height, width), np.float32) if confidence else None
+    _check(lib().oracle_blend(width, height, blend_px, n, _ptr(ext), dp, cp, _ptr(out_d), _ptr(out_c)))
+    return out_d, out_c
+
+
+def lr_mismatch(disp, other, sign, max_diff):
+    """1 where a finite disparity fails the left-right check against `other`
+    (x_o = lround(x + sign * d))."""
+    disp = _c(disp, np.float32)
+    other = _c(other, np.float32)
+    h, w = disp.shape
+    out = np.empty((h, w), np.uint8)
+    _check(lib().oracle_lr_mismatch(_ptr(disp), w, h, _ptr(other), other.shape[1], sign, max_diff, _ptr(out)))
+    return out
+
+
 def match_stereo(left, right, left_valid=None, right_valid=None, dmin=0, nd=64, options=None, threads=None):
     """match_stereo (stereo_matcher.cpp:1023-1095) with the level-0 integer outputs."""
     left = _c(left, np.uint16)

@@ -10,6 +10,8 @@ import enum
 
 R3DG_HOST = 0
 R3DG_DEVICE = 1
+R3DG_FORMAT_NATIVE = 0
+R3DG_FORMAT_U8X257 = 1
 
 
 class Status(enum.IntEnum):
@@ -33,7 +35,7 @@ class SubpixelMethod(enum.IntEnum):
 
 class CImage(ctypes.Structure):
     _fields_ = [("data", ctypes.c_void_p), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
-                ("stride", ctypes.c_int64), ("memory", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("stride", ctypes.c_int64), ("memory", ctypes.c_int32), ("format", ctypes.c_int32)]
 
 
 class COptions(ctypes.Structure):

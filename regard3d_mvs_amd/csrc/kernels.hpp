@@ -18,6 +18,9 @@ void launch_pyramid_u16(const std::uint16_t* in, std::int64_t is, int w, int h, 
                         std::uint16_t* out, std::int64_t os, hipStream_t s);
 void launch_pyramid_u8(const std::uint8_t* in, std::int64_t is, int w, int h, std::uint32_t* tmp, std::uint8_t* out,
                        std::int64_t os, bool threshold, hipStream_t s);
+// uint8 image to uint16 = u8 * 257 (io/image_io.hpp:25-26).
+void launch_widen_u8(const std::uint8_t* in, std::int64_t is, int w, int h, std::uint16_t* out, std::int64_t os,
+                     hipStream_t s);
 void launch_mirror_u8(const std::uint8_t* in, std::int64_t is, int w, int h, std::uint8_t* out, std::int64_t os,
                       hipStream_t s);
 // rows[y] = {first, last} valid column (mask >= 128); mask == nullptr: {0, w - 1}.
@@ -139,5 +142,44 @@ void launch_pack_rows(const std::int32_t* base, const std::int32_t* partner, int
 // Blends the tiles of one job into a full-size map.
 void launch_blend(GridDesc g, const TileDesc* tiles, const float* tdisp, const float* tconf, const std::int16_t* tlab,
                   float* disp, float* conf, std::int16_t* lab, hipStream_t s);
+
+// ---- row kernels of the reference's interface (census.hpp:25-29, sgm_kernels.hpp:41-97)
+// rows: 7 rows of n + 8 elements (image rows y - 3 .. y + 3, padded by 4 replicated columns).
+void launch_census_row(const std::uint16_t* rows, std::uint64_t n, std::uint64_t* out, hipStream_t s);
+// rev holds the caller's rev[rmin ..]; offsets are std::size_t of the caller's RowLabels.
+void launch_census_costs(const std::uint64_t* left, const std::uint64_t* rev, std::int64_t rmin, std::uint64_t width,
+                         const std::int32_t* lo, const std::uint16_t* num, const std::uint64_t* offset,
+                         std::int32_t origin, std::uint16_t* cost, hipStream_t s);
+// SgmRowArgs with device copies of every buffer; start is the zero start
+// buffer (scratch[0 .. path_stride(max_num))).
+struct SgmRowDev {
+  const std::uint16_t* cost;
+  const std::uint16_t* p2;        // [4][width]
+  const std::uint16_t* prev;      // [3][prev path size]
+  const std::uint16_t* prev_min;  // [3][width]
+  std::uint16_t* cur;             // [3][path size]
+  std::uint16_t* cur_min;         // [3][width]
+  std::uint16_t* sum;
+  const std::uint16_t* start;
+  const std::int32_t* lo;
+  const std::uint16_t* num;
+  const std::uint64_t* offset;    // [width + 1]
+  const std::int32_t* prev_lo;
+  const std::uint16_t* prev_num;
+  const std::uint64_t* prev_offset;
+  std::uint64_t width;
+  std::uint64_t max_num;
+  std::uint32_t p1;
+  std::int32_t direction;
+  std::int32_t has_prev;
+  std::int32_t accumulate;
+};
+// part: one uint16 per cell (paths 1..3); hbuf: 2 * (max_num + 2) uint16.
+void launch_sgm_row(const SgmRowDev& a, std::uint16_t* part, std::uint16_t* hbuf, hipStream_t s);
+// rc / rb: the caller's right arrays at index imin, count entries (may be null).
+void launch_wta_row(const std::uint16_t* sum, std::uint64_t width, const std::int32_t* lo, const std::uint16_t* num,
+                    const std::uint64_t* offset, std::uint16_t* best, std::uint16_t* cost, std::uint16_t* second,
+                    std::int64_t shift, std::int64_t imin, std::uint64_t count, std::uint16_t* rc, std::uint16_t* rb,
+                    hipStream_t s);
 
 }  // namespace r3dg

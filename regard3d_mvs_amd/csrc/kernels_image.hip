@@ -871,6 +871,33 @@ void launch_combine(const CombinePair* pairs, const CombineGrid& g, float* depth
 }
 
 
+// 8 pixels per thread: one 8-byte load where the row allows it, one 16-byte store.
+__global__ __launch_bounds__(256) void k_widen_u8(const std::uint8_t* __restrict__ in, std::int64_t is, int w,
+                                                  std::uint16_t* __restrict__ out, std::int64_t os) {
+  const int x0 = 8 * (blockIdx.x * blockDim.x + threadIdx.x), y = blockIdx.y;
+  if (x0 >= w) return;
+  const std::uint8_t* src = in + std::int64_t(y) * is + x0;
+  std::uint16_t* dst = out + std::int64_t(y) * os + x0;
+  const bool vec = x0 + 8 <= w && (reinterpret_cast<std::uintptr_t>(src) & 7) == 0 &&
+                   (reinterpret_cast<std::uintptr_t>(dst) & 15) == 0;
+  if (vec) {
+    const uint2 v = *reinterpret_cast<const uint2*>(src);
+    auto wide = [](std::uint32_t b) {  // two bytes -> two u16 values * 257
+      const std::uint32_t lo = b & 0xFFU, hi = (b >> 8) & 0xFFU;
+      return (lo * 257U) | ((hi * 257U) << 16);
+    };
+    *reinterpret_cast<uint4*>(dst) = make_uint4(wide(v.x), wide(v.x >> 16), wide(v.y), wide(v.y >> 16));
+    return;
+  }
+  for (int k = 0; k < 8 && x0 + k < w; ++k) dst[k] = static_cast<std::uint16_t>(src[k] * 257U);
+}
+
+void launch_widen_u8(const std::uint8_t* in, std::int64_t is, int w, int h, std::uint16_t* out, std::int64_t os,
+                     hipStream_t s) {
+  if (w <= 0 || h <= 0) return;
+  k_widen_u8<<<dim3((w + 8 * 256 - 1) / (8 * 256), h), 256, 0, s>>>(in, is, w, out, os);
+}
+
 void launch_mirror_u8(const std::uint8_t* in, std::int64_t is, int w, int h, std::uint8_t* out, std::int64_t os,
                       hipStream_t s) {
   k_mirror<std::uint8_t><<<dim3((w + 255) / 256, h), 256, 0, s>>>(in, is, w, h, out, os);

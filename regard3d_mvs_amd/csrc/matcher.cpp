@@ -106,10 +106,23 @@ DImg<T> view_of(const DImg<T>& o) {
 
 // A caller image on the device: device views are used in place, host images
 // are uploaded.  Empty views stay empty.
+// Host memory the library's runtime page-locked (r3dg_host_alloc): copied by
+// DMA directly, without a staging copy.
+bool host_pinned(const void* p) {
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeHost;
+}
+
 template <typename T>
 DImg<T> input_img(Context& c, const r3dg_image& v, const char* name) {
   if (view_empty(v)) return DImg<T>{};
   if (v.stride < v.width) fail(R3DG_ERROR_INVALID_ARGUMENT, std::string(name) + ": stride smaller than width");
+  if (v.format != R3DG_FORMAT_NATIVE)
+    fail(R3DG_ERROR_INVALID_ARGUMENT, std::string(name) + ": this input takes no format conversion");
   if (v.memory == R3DG_DEVICE) {
     DImg<T> d;
     d.p = static_cast<T*>(v.data);
@@ -120,7 +133,8 @@ DImg<T> input_img(Context& c, const r3dg_image& v, const char* name) {
   }
   DImg<T> d = alloc_img<T>(c, v.width, v.height);
   const std::size_t row = std::size_t(v.width) * sizeof(T), bytes = row * std::size_t(v.height);
-  void* pin = bytes >= (std::size_t{1} << 20) ? c.pinned.get(bytes) : nullptr;
+  const bool direct = bytes >= (std::size_t{1} << 16) && host_pinned(v.data);
+  void* pin = !direct && bytes >= (std::size_t{1} << 20) ? c.pinned.get(bytes) : nullptr;
   if (pin != nullptr) {
     // Stage through pinned memory (released at the next stream sync).
     c.staged_blocks.push_back(pin);
@@ -134,6 +148,20 @@ DImg<T> input_img(Context& c, const r3dg_image& v, const char* name) {
   return d;
 }
 
+// A uint16 image input: uint8 data (R3DG_FORMAT_U8X257) is transferred as
+// bytes and widened to u8 * 257 on the device.
+DImg<std::uint16_t> input_u16(Context& c, const r3dg_image& v, const char* name) {
+  if (view_empty(v) || v.format == R3DG_FORMAT_NATIVE) return input_img<std::uint16_t>(c, v, name);
+  if (v.format != R3DG_FORMAT_U8X257) fail(R3DG_ERROR_INVALID_ARGUMENT, std::string(name) + ": unknown format");
+  r3dg_image bytes = v;
+  bytes.format = R3DG_FORMAT_NATIVE;
+  const DImg<std::uint8_t> u = input_img<std::uint8_t>(c, bytes, name);
+  DImg<std::uint16_t> d = alloc_img<std::uint16_t>(c, v.width, v.height);
+  launch_widen_u8(u.p, u.s, u.w, u.h, d.p, d.s, c.stream);
+  hip_check(hipGetLastError(), "kernel launch");
+  return d;  // u's block returns to the arena; later users are ordered after the widening on the stream
+}
+
 template <typename T>
 void output_img(Context& c, const r3dg_image& v, const T* src, int w, int h, std::int64_t ss, const char* name) {
   if (view_empty(v)) return;
@@ -143,7 +171,7 @@ void output_img(Context& c, const r3dg_image& v, const T* src, int w, int h, std
                                           std::to_string(h));
   if (v.stride < v.width) fail(R3DG_ERROR_INVALID_ARGUMENT, std::string(name) + ": stride smaller than width");
   const std::size_t row = std::size_t(w) * sizeof(T), bytes = row * std::size_t(h);
-  if (v.memory != R3DG_DEVICE && bytes >= (std::size_t{1} << 20)) {
+  if (v.memory != R3DG_DEVICE && bytes >= (std::size_t{1} << 20) && !host_pinned(v.data)) {
     if (void* pin = c.pinned.get(bytes)) {
       // Into pinned memory; copied to the caller's buffer at the next sync.
       c.staged_blocks.push_back(pin);
@@ -180,6 +208,12 @@ void upload(Context& c, DBuf<T>& dst, const std::vector<T>& src) {
 
 void sync(Context& c) { c.sync(); }
 void check_launch() { hip_check(hipGetLastError(), "kernel launch"); }
+
+// Cancellation point: the context's flag or the caller's poll function.
+void poll_cancel(const Context& c) {
+  if (c.cancel.load() != 0 || (c.cancel_poll != nullptr && c.cancel_poll(c.cancel_user) != 0))
+    fail(R3DG_ERROR_CANCELLED, "cancelled");
+}
 
 // ------------------------------------------------------------------ stages
 DImg<std::uint64_t> census(Context& c, const DImg<std::uint16_t>& img) {
@@ -700,6 +734,7 @@ std::vector<TilesOut> match_tiles(Context& c, std::vector<Job>& jobs, const r3dg
   DBuf<std::int16_t> tlab;
   if (any_labels) tlab = DBuf<std::int16_t>(c.arena, std::max<std::uint64_t>(total_px, 1));
   for (const auto& [t0, t1] : chunk_tiles(c, A)) {
+    poll_cancel(c);
     Chunk ch = aggregate_tiles(c, A, t0, t1, false);
     // Selection into per-tile maps.
     const StageTimer timer(c, R3DG_STAGE_SELECT);
@@ -998,7 +1033,7 @@ MatchResult match_stereo(Context& c, View& Lv, View& Rv, int dmin, int nd, const
   P.want_debug = want_debug;
   start_pair(c, P, o);
   while (P.next >= 0) {
-    if (c.cancel.load() != 0) fail(R3DG_ERROR_CANCELLED, "cancelled");
+    poll_cancel(c);
     match_step(c, {&P}, o);
   }
   return std::move(P.res);
@@ -1016,7 +1051,7 @@ void match_batch(Context& c, const std::vector<PairRun*>& pairs, const r3dg_opti
     top = std::max(top, P->levels);
   }
   for (int l = top; l >= 0; --l) {
-    if (c.cancel.load() != 0) fail(R3DG_ERROR_CANCELLED, "cancelled");
+    poll_cancel(c);
     std::vector<PairRun*> part;
     for (PairRun* P : pairs)
       if (P->next == l) part.push_back(P);
@@ -1220,7 +1255,7 @@ class Server {
       return it->second;
     }
     auto v = std::make_shared<View>();
-    v->img = input_img<std::uint16_t>(c_, img, name);
+    v->img = input_u16(c_, img, name);
     v->mask = input_img<std::uint8_t>(c_, mask, "valid");
     return v;
   }
@@ -1544,11 +1579,76 @@ void depth_header(ByteReader& r, r3dg_depth_map& m, const char* path) {
 
 }  // namespace r3dg
 
+// ---- helpers of the row-kernel entry points
+namespace r3dg {
+namespace {
+
+template <typename T>
+DBuf<T> to_device(Context& c, const T* src, std::size_t n) {
+  DBuf<T> d(c.arena, std::max<std::size_t>(n, 1));
+  if (n > 0) hip_check(hipMemcpyAsync(d.get(), src, n * sizeof(T), hipMemcpyHostToDevice, c.stream), "upload");
+  return d;
+}
+
+template <typename T>
+void to_host(Context& c, T* dst, const DBuf<T>& src, std::size_t n) {
+  if (n > 0) hip_check(hipMemcpyAsync(dst, src.get(), n * sizeof(T), hipMemcpyDeviceToHost, c.stream), "download");
+}
+
+// Checks a RowLabels of `width` pixels so that every kernel access stays in
+// its buffers: offset[x] + num[x] <= offset[width], num[x] <= max_num (0: no
+// bound).  Returns offset[width].
+std::size_t check_labels(const r3dg_row_labels* L, std::size_t width, std::size_t max_num, const char* name) {
+  if (width == 0) return 0;
+  if (L == nullptr || L->lo == nullptr || L->num == nullptr || L->offset == nullptr)
+    fail(R3DG_ERROR_INVALID_ARGUMENT, std::string(name) + ": null labels");
+  const std::size_t total = L->offset[width];
+  for (std::size_t x = 0; x < width; ++x) {
+    if (L->offset[x] > total || total - L->offset[x] < L->num[x])
+      fail(R3DG_ERROR_INVALID_ARGUMENT, std::string(name) + ": labels of pixel " + std::to_string(x) +
+                                            " run past offset[width]");
+    if (max_num != 0 && L->num[x] > max_num)
+      fail(R3DG_ERROR_INVALID_ARGUMENT, std::string(name) + ": pixel " + std::to_string(x) + " has more than max_num labels");
+  }
+  if (total > (std::size_t{1} << 40)) fail(R3DG_ERROR_INVALID_ARGUMENT, std::string(name) + ": row too large");
+  return total;
+}
+
+DBuf<std::uint64_t> offsets_to_device(Context& c, const std::size_t* off, std::size_t width) {
+  std::vector<std::uint64_t> v(off, off + width + 1);
+  DBuf<std::uint64_t> d;
+  upload(c, d, v);
+  return d;
+}
+
+}  // namespace
+}  // namespace r3dg
+
 using namespace r3dg;
 
 extern "C" {
 
 int32_t r3dg_abi_version(void) { return R3DG_ABI_VERSION; }
+
+int32_t r3dg_host_alloc(size_t bytes, void** out) {
+  return guarded(nullptr, [&] {
+    if (out == nullptr) fail(R3DG_ERROR_INVALID_ARGUMENT, "host_alloc: null output");
+    *out = nullptr;
+    void* p = nullptr;
+    const hipError_t e = hipHostMalloc(&p, std::max<size_t>(bytes, 1), hipHostMallocDefault);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      fail(R3DG_ERROR_OUT_OF_MEMORY, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    }
+    *out = p;
+  });
+}
+
+int32_t r3dg_host_free(void* p) {
+  return guarded(nullptr, [&] {
+    if (p != nullptr) hip_check(hipHostFree(p), "hipHostFree");
+  });
+}
 
 void r3dg_default_options(r3dg_options* o) {
   if (o == nullptr) return;
@@ -1622,6 +1722,175 @@ int32_t r3dg_cancel(r3dg_ctx* h, int32_t requested) {
   if (h == nullptr) return R3DG_ERROR_INVALID_ARGUMENT;
   reinterpret_cast<Context*>(h)->cancel.store(requested != 0 ? 1 : 0);
   return R3DG_OK;
+}
+
+int32_t r3dg_set_cancel_poll(r3dg_ctx* h, int32_t (*poll)(void*), void* user) {
+  return with_ctx(h, [&](Context& c) {
+    c.cancel_poll = poll;
+    c.cancel_user = user;
+  });
+}
+
+
+int32_t r3dg_census_row(r3dg_ctx* h, const uint16_t* const rows[7], uint64_t* out, size_t n) {
+  return with_ctx(h, [&](Context& c) {
+    if (n == 0) return;
+    if (rows == nullptr || out == nullptr) fail(R3DG_ERROR_INVALID_ARGUMENT, "census_row: null pointer");
+    const std::size_t stride = n + 8;
+    std::vector<std::uint16_t> packed(7 * stride);
+    for (int j = 0; j < 7; ++j) {
+      if (rows[j] == nullptr) fail(R3DG_ERROR_INVALID_ARGUMENT, "census_row: null row");
+      std::memcpy(packed.data() + std::size_t(j) * stride, rows[j], stride * sizeof(std::uint16_t));
+    }
+    DBuf<std::uint16_t> drows;
+    upload(c, drows, packed);
+    DBuf<std::uint64_t> dout(c.arena, n);
+    launch_census_row(drows.get(), n, dout.get(), c.stream);
+    check_launch();
+    to_host(c, out, dout, n);
+    sync(c);
+  });
+}
+
+int32_t r3dg_census_costs(r3dg_ctx* h, const uint64_t* left, const uint64_t* rev, size_t width,
+                          const r3dg_row_labels* labels, int32_t rev_origin, uint16_t* cost) {
+  return with_ctx(h, [&](Context& c) {
+    if (width == 0) return;
+    if (left == nullptr || rev == nullptr || cost == nullptr) fail(R3DG_ERROR_INVALID_ARGUMENT, "census_costs: null pointer");
+    const std::size_t cells = check_labels(labels, width, 0, "census_costs");
+    std::int64_t rmin = INT64_MAX, rmax = INT64_MIN;
+    for (std::size_t x = 0; x < width; ++x) {
+      if (labels->num[x] == 0) continue;
+      const std::int64_t b = std::int64_t(width) - 1 - std::int64_t(x) + labels->lo[x] - rev_origin;
+      rmin = std::min(rmin, b);
+      rmax = std::max(rmax, b + labels->num[x]);
+    }
+    if (rmin > rmax) return;  // no labels
+    const DBuf<std::uint64_t> dl = to_device(c, left, width);
+    const DBuf<std::uint64_t> dr = to_device(c, rev + rmin, std::size_t(rmax - rmin));
+    const DBuf<std::int32_t> dlo = to_device(c, labels->lo, width);
+    const DBuf<std::uint16_t> dnum = to_device(c, labels->num, width);
+    const DBuf<std::uint64_t> doff = offsets_to_device(c, labels->offset, width);
+    DBuf<std::uint16_t> dcost(c.arena, std::max<std::size_t>(cells, 1));
+    launch_census_costs(dl.get(), dr.get(), rmin, width, dlo.get(), dnum.get(), doff.get(), rev_origin, dcost.get(),
+                        c.stream);
+    check_launch();
+    to_host(c, cost, dcost, cells);
+    sync(c);
+  });
+}
+
+int32_t r3dg_sgm_row(r3dg_ctx* h, const r3dg_sgm_row_args* a) {
+  return with_ctx(h, [&](Context& c) {
+    if (a == nullptr) fail(R3DG_ERROR_INVALID_ARGUMENT, "sgm_row: null arguments");
+    const std::size_t w = a->width;
+    if (w == 0) return;
+    if (a->cost == nullptr || a->p2 == nullptr || a->cur == nullptr || a->cur_min == nullptr ||
+        a->scratch == nullptr || a->sum == nullptr || (a->has_prev && (a->prev == nullptr || a->prev_min == nullptr)))
+      fail(R3DG_ERROR_INVALID_ARGUMENT, "sgm_row: null buffer");
+    if (a->max_num == 0 || a->max_num > 65535) fail(R3DG_ERROR_INVALID_ARGUMENT, "sgm_row: max_num out of range");
+    const std::size_t cells = check_labels(&a->labels, w, a->max_num, "sgm_row");
+    const std::size_t path = cells + w * 16;
+    std::size_t pcells = 0, ppath = 0;
+    if (a->has_prev) {
+      pcells = check_labels(&a->prev_labels, w, a->max_num, "sgm_row (preceding row)");
+      ppath = pcells + w * 16;
+    }
+    const std::size_t stride = a->max_num + 16;
+    SgmRowDev d{};
+    const DBuf<std::uint16_t> dcost = to_device(c, a->cost, cells), dp2 = to_device(c, a->p2, 4 * w),
+                              dstart = to_device(c, static_cast<const std::uint16_t*>(a->scratch), stride);
+    DBuf<std::uint16_t> dcur = to_device(c, static_cast<const std::uint16_t*>(a->cur), 3 * path);
+    DBuf<std::uint16_t> dsum = a->accumulate ? to_device(c, static_cast<const std::uint16_t*>(a->sum), cells)
+                                             : DBuf<std::uint16_t>(c.arena, std::max<std::size_t>(cells, 1));
+    DBuf<std::uint16_t> dcur_min(c.arena, 3 * w), part(c.arena, std::max<std::size_t>(cells, 1)),
+        hbuf(c.arena, 2 * (a->max_num + 2));
+    const DBuf<std::int32_t> dlo = to_device(c, a->labels.lo, w);
+    const DBuf<std::uint16_t> dnum = to_device(c, a->labels.num, w);
+    const DBuf<std::uint64_t> doff = offsets_to_device(c, a->labels.offset, w);
+    DBuf<std::uint16_t> dprev, dprev_min, dpnum;
+    DBuf<std::int32_t> dplo;
+    DBuf<std::uint64_t> dpoff;
+    if (a->has_prev) {
+      dprev = to_device(c, a->prev, 3 * ppath);
+      dprev_min = to_device(c, a->prev_min, 3 * w);
+      dplo = to_device(c, a->prev_labels.lo, w);
+      dpnum = to_device(c, a->prev_labels.num, w);
+      dpoff = offsets_to_device(c, a->prev_labels.offset, w);
+    }
+    d.cost = dcost.get();
+    d.p2 = dp2.get();
+    d.prev = dprev.get();
+    d.prev_min = dprev_min.get();
+    d.cur = dcur.get();
+    d.cur_min = dcur_min.get();
+    d.sum = dsum.get();
+    d.start = dstart.get();
+    d.lo = dlo.get();
+    d.num = dnum.get();
+    d.offset = doff.get();
+    d.prev_lo = dplo.get();
+    d.prev_num = dpnum.get();
+    d.prev_offset = dpoff.get();
+    d.width = w;
+    d.max_num = a->max_num;
+    d.p1 = a->p1;
+    d.direction = a->direction > 0 ? 1 : -1;
+    d.has_prev = a->has_prev ? 1 : 0;
+    d.accumulate = a->accumulate ? 1 : 0;
+    launch_sgm_row(d, part.get(), hbuf.get(), c.stream);
+    check_launch();
+    to_host(c, a->cur, dcur, 3 * path);
+    to_host(c, a->cur_min, dcur_min, 3 * w);
+    to_host(c, a->sum, dsum, cells);
+    sync(c);
+  });
+}
+
+int32_t r3dg_wta_row(r3dg_ctx* h, const uint16_t* sum, size_t width, const r3dg_row_labels* labels, uint16_t* best,
+                     uint16_t* best_cost, uint16_t* second, uint16_t* right_cost, uint16_t* right_best,
+                     ptrdiff_t right_shift) {
+  return with_ctx(h, [&](Context& c) {
+    if (width == 0) return;
+    if (sum == nullptr || best == nullptr || best_cost == nullptr || second == nullptr)
+      fail(R3DG_ERROR_INVALID_ARGUMENT, "wta_row: null pointer");
+    if ((right_cost == nullptr) != (right_best == nullptr))
+      fail(R3DG_ERROR_INVALID_ARGUMENT, "wta_row: right_cost and right_best go together");
+    const std::size_t cells = check_labels(labels, width, 65535, "wta_row");
+    for (std::size_t x = 0; x < width; ++x)
+      if (labels->num[x] == 0) fail(R3DG_ERROR_INVALID_ARGUMENT, "wta_row: a pixel without labels");
+    std::int64_t imin = INT64_MAX, imax = INT64_MIN;
+    if (right_cost != nullptr) {
+      for (std::size_t x = 0; x < width; ++x) {
+        const std::int64_t top = std::int64_t(x) - labels->lo[x] + right_shift;
+        imin = std::min(imin, top - (labels->num[x] - 1));
+        imax = std::max(imax, top);
+      }
+      if (imin < 0) fail(R3DG_ERROR_INVALID_ARGUMENT, "wta_row: right index below 0 (right_shift too small)");
+    }
+    const DBuf<std::uint16_t> dsum = to_device(c, sum, cells);
+    const DBuf<std::int32_t> dlo = to_device(c, labels->lo, width);
+    const DBuf<std::uint16_t> dnum = to_device(c, labels->num, width);
+    const DBuf<std::uint64_t> doff = offsets_to_device(c, labels->offset, width);
+    DBuf<std::uint16_t> db(c.arena, width), dc(c.arena, width), ds(c.arena, width), drc, drb;
+    std::size_t count = 0;
+    if (right_cost != nullptr) {
+      count = std::size_t(imax - imin + 1);
+      drc = to_device(c, static_cast<const std::uint16_t*>(right_cost) + imin, count);
+      drb = to_device(c, static_cast<const std::uint16_t*>(right_best) + imin, count);
+    }
+    launch_wta_row(dsum.get(), width, dlo.get(), dnum.get(), doff.get(), db.get(), dc.get(), ds.get(), right_shift,
+                   imin, count, drc.get(), drb.get(), c.stream);
+    check_launch();
+    to_host(c, best, db, width);
+    to_host(c, best_cost, dc, width);
+    to_host(c, second, ds, width);
+    if (count > 0) {
+      to_host(c, right_cost + imin, drc, count);
+      to_host(c, right_best + imin, drb, count);
+    }
+    sync(c);
+  });
 }
 
 int32_t r3dg_automatic_levels(int32_t width, int32_t height, int32_t num_disparities) {
@@ -1853,7 +2122,7 @@ int32_t r3dg_rectify(r3dg_ctx* h, int32_t dtype, r3dg_image src, const r3dg_came
 int32_t r3dg_census(r3dg_ctx* h, r3dg_image img, r3dg_image out) {
   return with_ctx(h, [&](Context& c) {
     if (view_empty(img)) return;
-    DImg<std::uint16_t> d = input_img<std::uint16_t>(c, img, "census input");
+    DImg<std::uint16_t> d = input_u16(c, img, "census input");
     DImg<std::uint64_t> r = census(c, d);
     output_img(c, out, r.p, r.w, r.h, r.s, "census output");
     sync(c);
@@ -1887,8 +2156,8 @@ int32_t r3dg_cost_volume(r3dg_ctx* h, r3dg_image left, r3dg_image right, int32_t
     r3dg_options tmp;
     const r3dg_options& o = opts_or_default(options, tmp);
     check_options(o);
-    DImg<std::uint16_t> L = input_img<std::uint16_t>(c, left, "left");
-    DImg<std::uint16_t> R = input_img<std::uint16_t>(c, right, "right");
+    DImg<std::uint16_t> L = input_u16(c, left, "left");
+    DImg<std::uint16_t> R = input_u16(c, right, "right");
     const DImg<std::uint8_t> none;
     check_inputs(L, R, none, none, num_disparities);
     const int D = align16(num_disparities);
@@ -1972,8 +2241,8 @@ int32_t r3dg_match(r3dg_ctx* h, r3dg_image left, r3dg_image right, r3dg_image le
     const r3dg_options& o = opts_or_default(options, tmp);
     check_options(o);
     View Lv, Rv;
-    Lv.img = input_img<std::uint16_t>(c, left, "left");
-    Rv.img = input_img<std::uint16_t>(c, right, "right");
+    Lv.img = input_u16(c, left, "left");
+    Rv.img = input_u16(c, right, "right");
     Lv.mask = input_img<std::uint8_t>(c, left_valid, "left_valid");
     Rv.mask = input_img<std::uint8_t>(c, right_valid, "right_valid");
     if (view_empty(disparity)) fail(R3DG_ERROR_INVALID_ARGUMENT, "match: empty disparity output");
@@ -2038,8 +2307,8 @@ int32_t r3dg_match_batch(r3dg_ctx* h, r3dg_pair* pairs, int32_t count, const r3d
       r3dg_pair& q = pairs[i];
       try {
         auto x = std::make_unique<Inputs>();
-        x->L.img = input_img<std::uint16_t>(c, q.left, "left");
-        x->R.img = input_img<std::uint16_t>(c, q.right, "right");
+        x->L.img = input_u16(c, q.left, "left");
+        x->R.img = input_u16(c, q.right, "right");
         x->L.mask = input_img<std::uint8_t>(c, q.left_valid, "left_valid");
         x->R.mask = input_img<std::uint8_t>(c, q.right_valid, "right_valid");
         if (view_empty(q.disparity)) fail(R3DG_ERROR_INVALID_ARGUMENT, "match: empty disparity output");
@@ -2228,7 +2497,7 @@ int32_t r3dg_view_put(r3dg_ctx* h, uint64_t view, r3dg_image image, r3dg_image v
     if (view == 0) fail(R3DG_ERROR_INVALID_ARGUMENT, "view: id 0 is reserved");
     if (view_empty(image)) fail(R3DG_ERROR_INVALID_ARGUMENT, "view: empty image");
     auto v = std::make_shared<View>();
-    v->img = input_img<std::uint16_t>(c, image, "view");
+    v->img = input_u16(c, image, "view");
     v->mask = input_img<std::uint8_t>(c, valid, "view valid");
     if (!v->mask.empty() && (v->mask.w != v->img.w || v->mask.h != v->img.h))
       fail(R3DG_ERROR_INVALID_ARGUMENT, "view: mask size differs from the image");

@@ -247,6 +247,9 @@ struct Context {
   Arena arena;
   std::mutex mutex;
   std::atomic<int> cancel{0};
+  // Caller's cancellation test (r3dg_set_cancel_poll), polled with `cancel`.
+  int32_t (*cancel_poll)(void*) = nullptr;
+  void* cancel_user = nullptr;
   std::string error;
   int32_t error_status = R3DG_OK;
   // Profiling: HIP event pairs around stages, read back at the end of a call.

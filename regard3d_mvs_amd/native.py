@@ -12,7 +12,7 @@ import re
 
 import numpy as np
 
-from ._abi import (R3DG_DEVICE, R3DG_HOST, STAGES, CImage, CMatchDebug, CMatchInfo, CPair, DisparityMap, Status, StereoOptions,
+from ._abi import (R3DG_DEVICE, R3DG_FORMAT_U8X257, R3DG_HOST, STAGES, CImage, CMatchDebug, CMatchInfo, CPair, DisparityMap, Status, StereoOptions,
                    align16)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -67,8 +67,11 @@ def _is_torch(a):
     return type(a).__module__.startswith("torch")
 
 
-def _image(a, dtype) -> tuple[CImage, object]:
-    """CImage view of a 2-D array; returns (view, keep-alive)."""
+def _image(a, dtype, fmt=0) -> tuple[CImage, object]:
+    """CImage view of a 2-D array; returns (view, keep-alive).  A torch GPU
+    tensor is used in place: torch's current stream on its device is
+    synchronised first, so that the library (on its own HIP stream) sees
+    finished inputs and never writes an output block torch work still uses."""
     if a is None:
         return CImage(None, 0, 0, 0, R3DG_HOST, 0), None
     if _is_torch(a):
@@ -80,7 +83,9 @@ def _image(a, dtype) -> tuple[CImage, object]:
         if a.dim() != 2 or a.stride(1) != 1:
             raise ValueError("expected a 2-D tensor with unit column stride")
         mem = R3DG_DEVICE if a.is_cuda else R3DG_HOST
-        return CImage(ctypes.c_void_p(a.data_ptr()), a.shape[1], a.shape[0], a.stride(0), mem, 0), a
+        if a.is_cuda:
+            torch.cuda.current_stream(a.device).synchronize()
+        return CImage(ctypes.c_void_p(a.data_ptr()), a.shape[1], a.shape[0], a.stride(0), mem, fmt), a
     arr = np.asarray(a)
     if arr.dtype != np.dtype(dtype):
         arr = arr.astype(dtype)
@@ -89,7 +94,55 @@ def _image(a, dtype) -> tuple[CImage, object]:
     if arr.strides[1] != arr.itemsize or arr.strides[0] % arr.itemsize != 0:
         arr = np.ascontiguousarray(arr)
     return CImage(arr.ctypes.data_as(ctypes.c_void_p), arr.shape[1], arr.shape[0], arr.strides[0] // arr.itemsize,
-                  R3DG_HOST, 0), arr
+                  R3DG_HOST, fmt), arr
+
+
+def _is_u8(a):
+    return a is not None and str(a.dtype).split(".")[-1] == "uint8"
+
+
+def _image16(a):
+    """A uint16 image input; uint8 arrays are passed as bytes and widened to
+    u8 * 257 on the device (R3DG_FORMAT_U8X257)."""
+    if _is_u8(a):
+        return _image(a, np.uint8, R3DG_FORMAT_U8X257)
+    return _image(a, np.uint16)
+
+
+class HostBuffer:
+    """Page-locked host memory of the library's runtime (r3dg_host_alloc):
+    numpy arrays over it move to and from the device by DMA, unstaged."""
+
+    def __init__(self, nbytes):
+        self.ptr = ctypes.c_void_p()
+        _global_check(lib().r3dg_host_alloc(ctypes.c_size_t(max(int(nbytes), 1)), ctypes.byref(self.ptr)))
+        self.nbytes = int(nbytes)
+
+    def array(self, shape, dtype):
+        n = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        if n > self.nbytes:
+            raise ValueError("host buffer too small")
+        buf = (ctypes.c_char * max(n, 1)).from_address(self.ptr.value)
+        a = np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
+        return a
+
+    def close(self):
+        if self.ptr:
+            lib().r3dg_host_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def host_empty(shape, dtype):
+    """(array, owner): an uninitialised numpy array in page-locked memory; keep
+    `owner` alive while the array is used."""
+    b = HostBuffer(int(np.prod(shape)) * np.dtype(dtype).itemsize)
+    return b.array(shape, dtype), b
 
 
 def _like(ref, shape, dtype):
@@ -203,7 +256,7 @@ class Matcher:
 
     # census_transform (census.hpp:20)
     def census(self, img):
-        v, keep = _image(img, np.uint16)
+        v, keep = _image16(img)
         out = _like(img, _shape(img), np.uint64)
         o, keep2 = _image(out, np.uint64)
         self._check(lib().r3dg_census(self._ctx, v, o))
@@ -265,8 +318,8 @@ class Matcher:
     # detail::sgm_cost_volume (stereo_matcher.hpp:98-101)
     def cost_volume(self, left, right, dmin, nd, options=None, ranges=None):
         o = (options or StereoOptions()).to_c()
-        lv, k1 = _image(left, np.uint16)
-        rv, k2 = _image(right, np.uint16)
+        lv, k1 = _image16(left)
+        rv, k2 = _image16(right)
         h, w = _shape(left)
         if ranges is not None:
             lo, num = ranges
@@ -287,8 +340,8 @@ class Matcher:
               out=None):
         o = (options or StereoOptions()).to_c()
         h, w = _shape(left)
-        lv, k1 = _image(left, np.uint16)
-        rv, k2 = _image(right, np.uint16)
+        lv, k1 = _image16(left)
+        rv, k2 = _image16(right)
         mlv, k3 = _image(left_valid, np.uint8)
         mrv, k4 = _image(right_valid, np.uint8)
         if out is not None:
@@ -316,7 +369,8 @@ class Matcher:
 
     def match_batch(self, pairs, options=None, raise_errors=True):
         """r3dg_match_batch: `pairs` is a sequence of dicts with left, right and
-        optional left_valid, right_valid, dmin (0), nd (64).  Returns one
+        optional left_valid, right_valid, dmin (0), nd (64), out (disparity,
+        confidence buffers).  Returns one
         DisparityMap per pair (None for a pair that failed when raise_errors
         is False)."""
         o = (options or StereoOptions()).to_c()
@@ -326,9 +380,12 @@ class Matcher:
         for i, q in enumerate(pairs):
             left = q["left"]
             h, w = _shape(left)
-            disp = _like(left, (h, w), np.float32)
-            conf = _like(left, (h, w), np.float32)
-            views = [_image(left, np.uint16), _image(q["right"], np.uint16), _image(q.get("left_valid"), np.uint8),
+            if q.get("out") is not None:
+                disp, conf = q["out"]
+            else:
+                disp = _like(left, (h, w), np.float32)
+                conf = _like(left, (h, w), np.float32)
+            views = [_image16(left), _image16(q["right"]), _image(q.get("left_valid"), np.uint8),
                      _image(q.get("right_valid"), np.uint8), _image(disp, np.float32), _image(conf, np.float32)]
             keep.append(views)
             outs.append((disp, conf))
@@ -363,7 +420,7 @@ class Matcher:
         self._check(lib().r3dg_server_stop(self._ctx))
 
     def view_put(self, view_id, image, valid=None):
-        v, k1 = _image(image, np.uint16)
+        v, k1 = _image16(image)
         m, k2 = _image(valid, np.uint8)
         self._check(lib().r3dg_view_put(self._ctx, ctypes.c_uint64(view_id), v, m))
 
@@ -377,7 +434,7 @@ class Matcher:
         h, w = shape
         disp = np.empty((h, w), np.float32)
         conf = np.empty((h, w), np.float32)
-        views = [_image(left, np.uint16), _image(right, np.uint16), _image(left_valid, np.uint8),
+        views = [_image16(left), _image16(right), _image(left_valid, np.uint8),
                  _image(right_valid, np.uint8), _image(disp, np.float32), _image(conf, np.float32)]
         pair = CPair()
         pair.left, pair.right, pair.left_valid, pair.right_valid = (v[0] for v in views[:4])

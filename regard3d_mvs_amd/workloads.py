@@ -1,11 +1,15 @@
 """Mixed-size pair streams for the batched and served forms of the matcher.
 
-Pairs are crops of one large synthetic rectified pair (synth.make_pair, span
-512, slopes <= 0.3, 30% low-texture area): a crop keeps the same rows and the
-same first column in both images, so its disparities are those of the source.
-Sizes, aspect ratios, right-width jitter and disparity spans follow the
-scene (batch) and stream (serve) workloads; each left view is shared by up to
-four partners (its right images differ in width).
+Every view is its own synthetic left image (synth.make_view: value noise,
+mean 128, sigma 40) and each of its partners is a forward warp of it by its
+own piecewise-planar disparity field at the pair's own span
+(synth.make_partner: Voronoi planes with slopes <= 0.2, holes filled with
+fresh noise, N(0, 2) noise) -- the single-pair generator per pair.  Partners
+of a view share the view's Voronoi partition (its surfaces) and draw their
+own planes.  Sizes, aspect ratios (3:2 or 4:3), right-width jitter (+/-15%)
+and disparity spans follow the batched-scene and served-stream workloads.
+Images are kept as uint8; `left` / `right` give the matcher's uint16
+(u8 * 257, src/r3d/io/image_io.hpp:25-26).
 """
 from __future__ import annotations
 
@@ -16,66 +20,70 @@ import numpy as np
 
 from . import synth
 
-SOURCE_W, SOURCE_H = 6400, 4400  # holds 24 MP at 3:2 and at 4:3
-
 
 @dataclasses.dataclass
 class Request:
-    left: np.ndarray    # uint16 view into the source
-    right: np.ndarray
+    left8: np.ndarray   # uint8 left view (shared by the view's partners)
+    right8: np.ndarray  # uint8 partner image
     dmin: int
     nd: int
-    view: int           # id of the left view (shared by its partners)
+    view: int           # id of the left view
+    truth: np.ndarray | None = None  # int16 disparity of the left view for this partner
+
+    @property
+    def left(self) -> np.ndarray:
+        return self.left8.astype(np.uint16) * np.uint16(257)
+
+    @property
+    def right(self) -> np.ndarray:
+        return self.right8.astype(np.uint16) * np.uint16(257)
 
 
-def source_pair(seed: int = 31337) -> synth.Pair:
-    return synth.make_pair(SOURCE_W, SOURCE_H, 512, seed, slope=0.3, low_texture=0.3)
-
-
-def _crop_size(rng, mp: float) -> tuple[int, int]:
+def _size(rng, mp: float) -> tuple[int, int]:
     aspect = 1.5 if rng.random() < 0.5 else 4.0 / 3.0
     h = int(math.sqrt(mp * 1e6 / aspect))
     w = int(h * aspect)
-    return min(max(w, 64), SOURCE_W), min(max(h, 48), SOURCE_H)
+    return max(w, 64), max(h, 48)
 
 
-def make_requests(src: synth.Pair, views: int, partners, mp_range, span_range, seed: int, log_sizes: bool):
+def make_requests(views: int, partners, mp_range, span_range, seed: int, log_sizes: bool, keep_truth=False,
+                  view_base: int = 0):
     """`partners(rng)` gives the number of pairs of a view; sizes are uniform
     (log-uniform with log_sizes) in mp_range megapixels, spans log-uniform in
-    span_range."""
+    span_range, right widths the view's +/-15%."""
     rng = np.random.default_rng(seed)
     lo, hi = mp_range
     out = []
     for v in range(views):
         mp = math.exp(rng.uniform(math.log(lo), math.log(hi))) if log_sizes else rng.uniform(lo, hi)
-        w, h = _crop_size(rng, mp)
-        y0 = int(rng.integers(0, SOURCE_H - h + 1))
-        x0 = int(rng.integers(0, SOURCE_W - w + 1))
-        left = src.left[y0:y0 + h, x0:x0 + w]
+        w, h = _size(rng, mp)
+        vseed = int(rng.integers(0, 2 ** 62))
+        left8 = synth.make_view(w, h, vseed)
+        fields = synth.view_fields(left8, vseed + 1, int(math.ceil(w * 1.15)) + 1)
         for _ in range(partners(rng)):
-            wr = int(w * rng.uniform(0.85, 1.15))
-            wr = max(64, min(wr, SOURCE_W - x0))
-            right = src.right[y0:y0 + h, x0:x0 + wr]
+            wr = max(64, int(w * rng.uniform(0.85, 1.15)))
             nd = int(round(math.exp(rng.uniform(math.log(span_range[0]), math.log(span_range[1])))))
-            out.append(Request(left, right, 0, nd, v + 1))
+            right8, truth = synth.make_partner(left8, nd, int(rng.integers(0, 2 ** 62)), width=wr, fields=fields)
+            out.append(Request(left8, right8, 0, nd, view_base + v + 1, truth if keep_truth else None))
     return out
 
 
-def scene(src: synth.Pair, views: int = 40, seed: int = 3000):
-    """The batched scene: `views` views x 4 partners, 1.5-24 MP (3:2 or 4:3),
-    right widths +/-15%, spans log-uniform in 64..512."""
-    return make_requests(src, views, lambda rng: 4, (1.5, 24.0), (64, 512), seed, log_sizes=False)
+def scene(views: int = 40, mp_range=(1.5, 24.0), span_range=(64, 512), seed: int = 3000):
+    """The batched scene: `views` views x 4 partners, sizes uniform in
+    mp_range (1.5-24 MP), spans log-uniform in span_range (64..512)."""
+    return make_requests(views, lambda rng: 4, mp_range, span_range, seed, log_sizes=False)
 
 
-def stream(src: synth.Pair, requests: int, seed: int = 4000):
+def stream(requests: int, seed: int = 4000, mp_range=(0.3, 24.0), span_range=(32, 512)):
     """The served stream: sizes log-uniform in 0.3-24 MP, spans log-uniform in
     32..512, each view reused by 1 to 4 partners, cut to `requests` pairs."""
-    rng_views = max(1, requests // 2)
-    reqs = make_requests(src, rng_views, lambda rng: int(rng.integers(1, 5)), (0.3, 24.0), (32, 512), seed,
-                         log_sizes=True)
+    reqs = []
+    k = 0
     while len(reqs) < requests:
-        reqs += make_requests(src, rng_views, lambda rng: int(rng.integers(1, 5)), (0.3, 24.0), (32, 512),
-                              seed + len(reqs), log_sizes=True)
+        n = max(1, (requests - len(reqs) + 1) // 2)
+        reqs += make_requests(n, lambda rng: int(rng.integers(1, 5)), mp_range, span_range, seed + k,
+                              log_sizes=True, view_base=1 + max((r.view for r in reqs), default=0))
+        k += 1
     return reqs[:requests]
 
 

@@ -25,7 +25,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_host_only_entry_points():
     lib = native.lib()
-    assert lib.r3dg_abi_version() == 2
+    assert lib.r3dg_abi_version() == 3
     assert native.automatic_levels(6048, 4032, 512) == 4
     assert native.automatic_levels(640, 480, 64) == 1
     from regard3d_mvs_amd._abi import COptions, StereoOptions
