@@ -1,16 +1,19 @@
-// gfx950 kernels of the SGM core: tile label layout, the 8 path scans,
-// winner-takes-all selection and tile blending.
+// gfx950 kernels of the SGM core: tile label layout, matching costs, the 8
+// path scans, winner-takes-all selection and tile blending.
 //
-// Aggregation dataflow (HBM-bound, no MFMA: nothing here is a contraction):
-// one launch per path direction over every tile of every job.  A wave64 scans
-// four adjacent scanlines of one tile at once, 16 lanes per scanline, labels
-// across the 16 lanes (label j on lane j mod 16).  The predecessor's path costs
-// live in LDS (two uint16 buffers per scanline, re-aligned to the current
-// pixel's range by index arithmetic, labels outside the predecessor's range
-// read as 0xFFFF), the matching cost is recomputed on the fly as
-// popcount(CL ^ CR) from the Census images, and the min over labels is a
-// 16-lane butterfly.  The first path stores S, the others add to it (uint16
-// wrap-around, so the order of the paths does not matter).
+// Aggregation dataflow (HBM-bound, no MFMA: nothing here is a contraction).
+// k_cost writes the matching cost of every tile cell once per level as one
+// byte (popcount of the Census XOR, or invalid_cost).  Each path direction is
+// one persistent launch over every tile of a chunk: waves take groups of 8
+// adjacent scanlines from a work queue; a scanline is 8 lanes, each holding 4
+// or 8 consecutive labels of the current pixel as packed uint16 pairs, and a
+// pixel's labels are processed in segments (32 labels, then 64 at a time).
+// The predecessor's path costs sit in an LDS row with 0xFFFF sentinels around
+// its labels (a global scratch row for pixels of more than kNarrow labels),
+// the min over a pixel's labels is an 8-lane DPP reduction, and costs, S and
+// the pixel records are prefetched a few steps ahead.  The first path of a
+// stream writes its slice of S, the others add to it (uint16 wrap-around, so
+// the order of the paths does not matter); the selection sums the slices.
 //
 // Recurrence (sgm_kernels.cpp:176-228, 387-403; stereo_matcher.cpp:1280-1362):
 //   L(p,d) = C(p,d) + min(L(q,d), sat16(min(L(q,d-1), L(q,d+1)) + P1),

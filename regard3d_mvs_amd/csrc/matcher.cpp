@@ -583,6 +583,12 @@ Aggregation layout_jobs(Context& c, std::vector<Job>& jobs, const r3dg_options& 
   for (const Job& J : jobs) max_D = std::max(max_D, J.D);
   if (max_num > kMaxLabels)
     fail(R3DG_ERROR_INVALID_ARGUMENT, "stereo: more than " + std::to_string(kMaxLabels) + " labels per pixel are not supported on the GPU");
+  // The path scans address a tile's volume with 32-bit byte offsets (2 B per
+  // cell of S), so a tile holds fewer than 2^31 cells.
+  if (A.max_tile_cells >= (std::uint64_t{1} << 31))
+    fail(R3DG_ERROR_INVALID_ARGUMENT, "stereo: a tile of " + std::to_string(A.max_tile_cells) +
+                                          " cells exceeds the 2^31 cells a tile volume may hold on the GPU; "
+                                          "use a smaller tile_size or memory_budget");
   upload(c, dtiles, td);
   layout_timer.reset();
   return A;
@@ -2469,15 +2475,24 @@ int32_t r3dg_view_put(r3dg_ctx* h, uint64_t view, r3dg_image image, r3dg_image v
       return guarded(nullptr, [&] {
         if (view == 0) fail(R3DG_ERROR_INVALID_ARGUMENT, "view: id 0 is reserved");
         if (view_empty(image) || image.stride < image.width) fail(R3DG_ERROR_INVALID_ARGUMENT, "view: bad image");
+        if (image.format != R3DG_FORMAT_NATIVE && image.format != R3DG_FORMAT_U8X257)
+          fail(R3DG_ERROR_INVALID_ARGUMENT, "view: unknown format");
         Server::ViewOp op;
         op.id = view;
         op.w = image.width;
         op.h = image.height;
         op.img.resize(std::size_t(image.width) * std::size_t(image.height));
-        for (int y = 0; y < image.height; ++y)
-          std::memcpy(op.img.data() + std::size_t(y) * std::size_t(image.width),
-                      static_cast<const std::uint16_t*>(image.data) + std::size_t(y) * std::size_t(image.stride),
-                      std::size_t(image.width) * 2);
+        for (int y = 0; y < image.height; ++y) {
+          std::uint16_t* dst = op.img.data() + std::size_t(y) * std::size_t(image.width);
+          if (image.format == R3DG_FORMAT_U8X257) {
+            // uint8 rows widened to u8 * 257 here; the loop uploads uint16.
+            const auto* src = static_cast<const std::uint8_t*>(image.data) + std::size_t(y) * std::size_t(image.stride);
+            for (int x = 0; x < image.width; ++x) dst[x] = static_cast<std::uint16_t>(src[x] * 257U);
+          } else {
+            std::memcpy(dst, static_cast<const std::uint16_t*>(image.data) + std::size_t(y) * std::size_t(image.stride),
+                        std::size_t(image.width) * 2);
+          }
+        }
         if (!view_empty(valid)) {
           if (valid.width != image.width || valid.height != image.height || valid.stride < valid.width)
             fail(R3DG_ERROR_INVALID_ARGUMENT, "view: mask size differs from the image");

@@ -88,12 +88,16 @@ def test_batch_and_views_take_u8(gpu, case):
     l8, r8 = (p.left // 257).astype(np.uint8), (p.right // 257).astype(np.uint8)
     got = gpu.match_batch([dict(left=l8, right=r8, dmin=p.dmin, nd=p.nd)], o)[0]
     same_bits(got.disparity, ref.disparity)
+    gpu.view_put(6, l8)  # before the loop runs: uploaded and widened on the device
     gpu.serve_start(o)
     try:
-        gpu.view_put(5, l8)
-        t = gpu.submit(l8.shape, p.dmin, p.nd, right=r8, left_view=5)
-        g = gpu.wait(t)
+        gpu.view_put(5, l8)  # while the loop runs: widened on the host, uploaded by the loop
+        t5 = gpu.submit(l8.shape, p.dmin, p.nd, right=r8, left_view=5)
+        t6 = gpu.submit(l8.shape, p.dmin, p.nd, right=r8, left_view=6)
+        g5, g6 = gpu.wait(t5), gpu.wait(t6)
     finally:
         gpu.serve_stop()
         gpu.view_evict(5)
-    same_bits(g.disparity, ref.disparity)
+        gpu.view_evict(6)
+    same_bits(g5.disparity, ref.disparity)
+    same_bits(g6.disparity, ref.disparity)
