@@ -269,7 +269,7 @@ def bench_batch(args) -> int:
     from regard3d_mvs_amd.native import Matcher
 
     t = time.perf_counter()
-    reqs = workloads.scene(views=args.views)
+    reqs = workloads.scene(views=args.views, workers=args.workers)
     log(f"bench: {len(reqs)} pairs generated in {time.perf_counter() - t:.1f} s; "
         f"{sum(r.left8.size for r in reqs) / len(reqs) / 1e6:.1f} MP mean")
     o = StereoOptions()
@@ -322,7 +322,7 @@ def bench_batch(args) -> int:
         "gcell_path_per_s": cells * 8 / t_batch / 1e9, "plain_gcell_path_per_s": cells * 8 / t_plain / 1e9,
         "host_in_out": {"batched_pairs_per_s": len(reqs) / th_batch, "plain_pairs_per_s": len(reqs) / th_plain},
         "config": {"workload": f"{args.views} views x 4 partners, 1.5-24 MP (3:2 or 4:3), right widths +/-15%, "
-                               "spans log-uniform 64..512, crops of a 6400x4400 span-512 synthetic pair",
+                               "spans log-uniform 64..512, every pair from the single-pair generator at its own span",
                    "inputs": "uint16 resident in HBM (value); host uint16 in / host float out (host_in_out)"},
     }
     print(json.dumps(out), flush=True)
@@ -339,17 +339,25 @@ def bench_serve(args) -> int:
     from regard3d_mvs_amd import StereoOptions, workloads
     from regard3d_mvs_amd.native import Matcher
 
-    reqs = workloads.stream(args.requests)
-    log(f"bench: {len(reqs)} requests, {sum(r.left.size for r in reqs) / len(reqs) / 1e6:.2f} MP mean, "
-        f"{len(set(r.view for r in reqs))} views")
+    import hashlib
+
+    def digest(a):
+        return hashlib.blake2b(np.ascontiguousarray(a).view(np.uint8), digest_size=16).digest()
+
+    t = time.perf_counter()
+    reqs = workloads.stream(args.requests, workers=args.workers)
+    log(f"bench: {len(reqs)} requests generated in {time.perf_counter() - t:.1f} s, "
+        f"{sum(r.left8.size for r in reqs) / len(reqs) / 1e6:.2f} MP mean, {len(set(r.view for r in reqs))} views")
     o = StereoOptions()
     m = Matcher(0)
-    m.match(reqs[0].left, reqs[0].right, None, None, reqs[0].dmin, reqs[0].nd, o)  # warm-up
+    m.match(reqs[0].left8, reqs[0].right8, None, None, reqs[0].dmin, reqs[0].nd, o)  # warm-up
     service, plain = [], []
-    for r in reqs:
+    for i, r in enumerate(reqs):
         s0 = time.perf_counter()
-        plain.append(m.match(r.left, r.right, None, None, r.dmin, r.nd, o).disparity)
+        plain.append(digest(m.match(r.left8, r.right8, None, None, r.dmin, r.nd, o).disparity))
         service.append(time.perf_counter() - s0)
+        if (i + 1) % 250 == 0:
+            log(f"bench: plain {i + 1}/{len(reqs)}")
     log(f"bench: plain service {sum(service):.2f} s ({len(reqs) / sum(service):.2f} pairs/s)")
     results = []
     for rate in [float(x) for x in args.rates.split(",")]:
@@ -366,10 +374,10 @@ def bench_serve(args) -> int:
                 if d > 0:
                     time.sleep(d)
                 if r.view not in put:
-                    m.view_put(r.view, r.left)
+                    m.view_put(r.view, r.left8)
                     put.add(r.view)
                 late[i] = time.perf_counter() - (t0 + a)
-                tickets[i] = m.submit(r.left.shape, r.dmin, r.nd, right=r.right, left_view=r.view)
+                tickets[i] = m.submit(r.left8.shape, r.dmin, r.nd, right=r.right8, left_view=r.view)
 
         th = threading.Thread(target=client)
         th.start()
@@ -379,8 +387,10 @@ def bench_serve(args) -> int:
                 time.sleep(0.0005)
             g = m.wait(tickets[i])
             lat.append(late[i] + g.seconds)  # scheduled arrival to completion
-            exact = exact and _same(g.disparity, plain[i])
+            exact = exact and digest(g.disparity) == plain[i]
             done = time.perf_counter()
+            if (i + 1) % 250 == 0:
+                log(f"bench: rate {rate}: {i + 1}/{len(reqs)} served")
         th.join()
         m.serve_stop()
         for v in put:
@@ -398,8 +408,10 @@ def bench_serve(args) -> int:
            "value": results[-1]["pairs_per_s"], "plain_capacity_pairs_per_s": len(reqs) / sum(service),
            "higher_is_better": True, "data": "synthetic", "dtype": "u16", "rates": results,
            "config": {"workload": f"{len(reqs)} requests per rate, Poisson arrivals, 0.3-24 MP log-uniform, "
-                                  "spans log-uniform 32..512, views reused by 1-4 partners, crops of a "
-                                  "6400x4400 span-512 synthetic pair", "hbm_budget_gb": args.hbm_budget}}
+                                  "spans log-uniform 32..512, each view reused by 1-4 partners, every pair from "
+                                  "the single-pair generator at its own span", "hbm_budget_gb": args.hbm_budget,
+                      "inputs": "host uint8 images in (views put once, right images per request), host float "
+                                "maps out; latency from scheduled arrival to result in host memory"}}
     print(json.dumps(out), flush=True)
     m.close()
     return 0
@@ -423,8 +435,9 @@ def main() -> int:
                          "device-to-device copy (calibrates the FETCH_SIZE / WRITE_SIZE counters); rectify: "
                          "rectify_image of the 24 MP image; combine: combine_pairs of 4 pairs on a 24 MP grid")
     ap.add_argument("--views", type=int, default=40, help="batch: views (4 partners each)")
-    ap.add_argument("--requests", type=int, default=300, help="serve: requests per rate")
-    ap.add_argument("--rates", default="5,20,50", help="serve: arrival rates (pairs/s)")
+    ap.add_argument("--requests", type=int, default=2000, help="serve: requests per rate")
+    ap.add_argument("--rates", default="5,20,50,200", help="serve: arrival rates (pairs/s; 200 saturates)")
+    ap.add_argument("--workers", type=int, default=16, help="batch / serve: processes generating the images")
     ap.add_argument("--hbm-budget", type=float, default=200.0, help="serve: admission budget (GB)")
     args = ap.parse_args()
     if args.gpus != 1:

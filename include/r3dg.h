@@ -180,6 +180,13 @@ int32_t r3dg_global_error(char* buf, size_t size);
  * stream and reports their durations in r3dg_match_info::stage_seconds. */
 int32_t r3dg_set_profiling(r3dg_ctx* ctx, int32_t enabled);
 
+/* Test hook, no counterpart in the reference: after `after` further device
+ * allocations of this context succeed, the next one fails once as if the
+ * device were out of memory (R3DG_ERROR_OUT_OF_MEMORY from the call that
+ * makes it); a negative value disarms it.  Lets a host test force the
+ * allocation-failure path of every call and of the request loop. */
+int32_t r3dg_inject_alloc_failure(r3dg_ctx* ctx, int64_t after);
+
 /* Cancellation flag polled between pyramid levels and tile stages; replaces
  * CancellationToken polling in parallel_for (core/thread_pool.cpp:41-70).
  * The flag stays set until it is cleared (requested = 0), as a token stays

@@ -1092,9 +1092,10 @@ int32_t guarded(Context* c, F&& f) {
   if (c != nullptr) {
     c->error = msg;
     c->error_status = status;
-    // Leave the stream idle so the context stays usable; staged outputs of
-    // the failed call are dropped.
-    (void)hipStreamSynchronize(c->stream);
+    // Leave the streams idle so the context stays usable (blocks the failed
+    // call released go back to the arena only once no kernel uses them);
+    // staged outputs of the failed call are dropped.
+    c->sync_all();
     (void)hipGetLastError();
     c->drop_staged();
   }
@@ -1145,12 +1146,18 @@ ViewCache& view_cache(Context& c) {
 // Census, mirrored copies, maps) and its largest level's volumes (S and
 // costs, 3 bytes per cell) with about 48 labels per pixel when ranges come
 // from a coarser level.  Volumes beyond the memory left are chunked anyway.
-std::uint64_t pair_footprint(const r3dg_pair& q, const r3dg_options& o) {
+// The part that cannot be chunked: images, pyramids, Census and maps.
+std::uint64_t pair_fixed_bytes(const r3dg_pair& q) {
   const std::uint64_t wl = std::uint64_t(std::max(q.left.width, 0)), wr = std::uint64_t(std::max(q.right.width, 0));
+  return 96 * (wl + wr) * std::uint64_t(std::max(q.left.height, 0));
+}
+
+std::uint64_t pair_footprint(const r3dg_pair& q, const r3dg_options& o) {
+  const std::uint64_t wl = std::uint64_t(std::max(q.left.width, 0));
   const std::uint64_t h = std::uint64_t(std::max(q.left.height, 0));
   const int levels = o.levels >= 0 ? o.levels : automatic_levels(q.left.width, q.left.height, q.num_disparities);
   const std::uint64_t labels = levels > 0 ? 48 : std::uint64_t(align16(std::max(q.num_disparities, 1)));
-  return 96 * (wl + wr) * h + 3 * 12 * wl * h * labels / 5;
+  return pair_fixed_bytes(q) + 3 * 12 * wl * h * labels / 5;
 }
 
 struct Request {
@@ -1206,6 +1213,14 @@ class Server {
     r->left_view = lv;
     r->right_view = rv;
     r->footprint = pair_footprint(*pair, o_);
+    // Refused at admission: a pair whose images, pyramids, Census and maps
+    // alone exceed the context's device budget can never run (its volumes
+    // could be chunked, those cannot).
+    const std::uint64_t fixed = pair_fixed_bytes(*pair);
+    if (fixed > c_.arena.budget())
+      fail(R3DG_ERROR_BUDGET, "serve: the pair needs at least " + std::to_string(fixed >> 20) +
+                                  " MiB of device memory, more than the context's budget of " +
+                                  std::to_string(c_.arena.budget() >> 20) + " MiB");
     r->submitted = std::chrono::steady_clock::now();
     pair->status = R3DG_PENDING;
     std::uint64_t t;
@@ -1722,6 +1737,10 @@ int32_t r3dg_global_error(char* buf, size_t size) { return copy_error(g_error, g
 
 int32_t r3dg_set_profiling(r3dg_ctx* h, int32_t enabled) {
   return with_ctx(h, [&](Context& c) { c.profiling = enabled != 0 ? 1 : 0; });
+}
+
+int32_t r3dg_inject_alloc_failure(r3dg_ctx* h, int64_t after) {
+  return with_ctx(h, [&](Context& c) { c.arena.inject_failure(after < 0 ? -1 : after); });
 }
 
 int32_t r3dg_cancel(r3dg_ctx* h, int32_t requested) {

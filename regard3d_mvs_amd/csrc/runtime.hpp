@@ -52,6 +52,8 @@ class Arena {
   ~Arena() { trim(); }
 
   void* alloc(std::size_t bytes) {
+    if (fail_after_ >= 0 && fail_after_-- == 0)
+      fail(R3DG_ERROR_OUT_OF_MEMORY, "hipMalloc: injected allocation failure (r3dg_inject_alloc_failure)");
     const std::size_t size = round(bytes);
     auto it = free_.lower_bound(size);
     if (it != free_.end() && it->first <= 2 * size) {
@@ -115,6 +117,9 @@ class Arena {
   // Bytes of blocks in use (held minus the cached free blocks).
   [[nodiscard]] std::uint64_t live() const { return live_bytes_; }
   [[nodiscard]] std::uint64_t budget() const { return budget_; }
+  // Test hook: the allocation after `n` more succeed fails once as if the
+  // device were out of memory (n < 0: none).
+  void inject_failure(std::int64_t n) { fail_after_ = n; }
 
  private:
   static std::size_t round(std::size_t b) {
@@ -126,6 +131,7 @@ class Arena {
   std::uint64_t held_ = 0;
   std::uint64_t live_bytes_ = 0;
   std::uint64_t budget_;
+  std::int64_t fail_after_ = -1;
 };
 
 // Cached pinned host blocks (hipHostMalloc), for staging transfers of host
@@ -298,6 +304,13 @@ struct Context {
   // Extra streams for path scans that run side by side (levels whose scans do
   // not fill the device); created on first use, destroyed with the context.
   std::array<hipStream_t, 3> side{};
+  // Waits for the context's stream and every side stream.
+  void sync_all() {
+    (void)hipStreamSynchronize(stream);
+    for (hipStream_t s : side) {
+      if (s != nullptr) (void)hipStreamSynchronize(s);
+    }
+  }
   hipStream_t side_stream(int i) {
     if (side[std::size_t(i)] == nullptr)
       hip_check(hipStreamCreateWithFlags(&side[std::size_t(i)], hipStreamNonBlocking), "hipStreamCreate");

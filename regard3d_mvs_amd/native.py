@@ -254,6 +254,11 @@ class Matcher:
     def cancel(self, requested=True):
         self._check(lib().r3dg_cancel(self._ctx, int(bool(requested))))
 
+    def inject_alloc_failure(self, after):
+        """Test hook: the device allocation after `after` more fails once
+        (R3DG_ERROR_OUT_OF_MEMORY); negative disarms it."""
+        self._check(lib().r3dg_inject_alloc_failure(self._ctx, ctypes.c_int64(int(after))))
+
     # census_transform (census.hpp:20)
     def census(self, img):
         v, keep = _image16(img)

@@ -46,35 +46,58 @@ def _size(rng, mp: float) -> tuple[int, int]:
     return max(w, 64), max(h, 48)
 
 
+def _make_view(spec):
+    """Images of one view and its partners from their drawn parameters."""
+    w, h, vseed, parts, keep_truth = spec
+    left8 = synth.make_view(w, h, vseed)
+    fields = synth.view_fields(left8, vseed + 1, int(math.ceil(w * 1.15)) + 1)
+    out = []
+    for wr, nd, pseed in parts:
+        right8, truth = synth.make_partner(left8, nd, pseed, width=wr, fields=fields)
+        out.append((right8, nd, truth if keep_truth else None))
+    return left8, out
+
+
 def make_requests(views: int, partners, mp_range, span_range, seed: int, log_sizes: bool, keep_truth=False,
-                  view_base: int = 0):
+                  view_base: int = 0, workers: int = 1):
     """`partners(rng)` gives the number of pairs of a view; sizes are uniform
     (log-uniform with log_sizes) in mp_range megapixels, spans log-uniform in
-    span_range, right widths the view's +/-15%."""
+    span_range, right widths the view's +/-15%.  The parameters are drawn in
+    order from one generator; the images of the views are made by `workers`
+    processes (the same images for any count)."""
     rng = np.random.default_rng(seed)
     lo, hi = mp_range
-    out = []
-    for v in range(views):
+    specs = []
+    for _ in range(views):
         mp = math.exp(rng.uniform(math.log(lo), math.log(hi))) if log_sizes else rng.uniform(lo, hi)
         w, h = _size(rng, mp)
         vseed = int(rng.integers(0, 2 ** 62))
-        left8 = synth.make_view(w, h, vseed)
-        fields = synth.view_fields(left8, vseed + 1, int(math.ceil(w * 1.15)) + 1)
+        parts = []
         for _ in range(partners(rng)):
             wr = max(64, int(w * rng.uniform(0.85, 1.15)))
             nd = int(round(math.exp(rng.uniform(math.log(span_range[0]), math.log(span_range[1])))))
-            right8, truth = synth.make_partner(left8, nd, int(rng.integers(0, 2 ** 62)), width=wr, fields=fields)
-            out.append(Request(left8, right8, 0, nd, view_base + v + 1, truth if keep_truth else None))
+            parts.append((wr, nd, int(rng.integers(0, 2 ** 62))))
+        specs.append((w, h, vseed, parts, keep_truth))
+    if workers > 1 and len(specs) > 1:
+        import multiprocessing as mp_
+        with mp_.get_context("spawn").Pool(min(workers, len(specs))) as pool:
+            made = pool.map(_make_view, specs, chunksize=1)
+    else:
+        made = [_make_view(sp) for sp in specs]
+    out = []
+    for v, (left8, parts) in enumerate(made):
+        for right8, nd, truth in parts:
+            out.append(Request(left8, right8, 0, nd, view_base + v + 1, truth))
     return out
 
 
-def scene(views: int = 40, mp_range=(1.5, 24.0), span_range=(64, 512), seed: int = 3000):
+def scene(views: int = 40, mp_range=(1.5, 24.0), span_range=(64, 512), seed: int = 3000, workers: int = 1):
     """The batched scene: `views` views x 4 partners, sizes uniform in
     mp_range (1.5-24 MP), spans log-uniform in span_range (64..512)."""
-    return make_requests(views, lambda rng: 4, mp_range, span_range, seed, log_sizes=False)
+    return make_requests(views, lambda rng: 4, mp_range, span_range, seed, log_sizes=False, workers=workers)
 
 
-def stream(requests: int, seed: int = 4000, mp_range=(0.3, 24.0), span_range=(32, 512)):
+def stream(requests: int, seed: int = 4000, mp_range=(0.3, 24.0), span_range=(32, 512), workers: int = 1):
     """The served stream: sizes log-uniform in 0.3-24 MP, spans log-uniform in
     32..512, each view reused by 1 to 4 partners, cut to `requests` pairs."""
     reqs = []
@@ -82,7 +105,7 @@ def stream(requests: int, seed: int = 4000, mp_range=(0.3, 24.0), span_range=(32
     while len(reqs) < requests:
         n = max(1, (requests - len(reqs) + 1) // 2)
         reqs += make_requests(n, lambda rng: int(rng.integers(1, 5)), mp_range, span_range, seed + k,
-                              log_sizes=True, view_base=1 + max((r.view for r in reqs), default=0))
+                              log_sizes=True, view_base=1 + max((r.view for r in reqs), default=0), workers=workers)
         k += 1
     return reqs[:requests]
 
