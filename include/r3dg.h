@@ -270,6 +270,19 @@ int32_t r3dg_view_evict(r3dg_ctx* ctx, uint64_t view);
 /* remove_speckles (stereo_matcher.hpp:89), in place on a float image. */
 int32_t r3dg_remove_speckles(r3dg_ctx* ctx, r3dg_image disparity, int32_t min_size, float max_diff);
 
+/* The speckle filter of r3dg_remove_speckles, allocating nothing.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The map is filtered in place and must be device memory,
+ * contiguous (stride == width), of at most R3DG_STATIC_SPECKLE_PIXELS pixels;
+ * other maps are R3DG_ERROR_INVALID_ARGUMENT.  The union-find scratch is a
+ * fixed 32 MiB device array per GPU, held until the process ends and shared
+ * by every call on that GPU: calls through this library take turns and return
+ * after their kernels finish, but work that other code queues on the same
+ * array at the same time would race.  Results equal r3dg_remove_speckles. */
+#define R3DG_STATIC_SPECKLE_PIXELS 4194304
+int32_t r3dg_remove_speckles_static(r3dg_ctx* ctx, r3dg_image disparity, int32_t min_size, float max_diff);
+
 /* median_filter_3x3 (stereo_matcher.hpp:91). */
 int32_t r3dg_median_filter_3x3(r3dg_ctx* ctx, r3dg_image disparity, r3dg_image out);
 

@@ -22,7 +22,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libr3dg.so")
 API_LIB = os.path.join(HERE, "libr3d_stereo.so")          # r3d:: C++ API over the C ABI
 API_TEST = os.path.join(ROOT, "tests", "cpp", "_build", "test_stereo_api")
-SOURCES = ["kernels_image.hip", "kernels_sgm.hip", "kernels_rows.hip", "matcher.cpp"]
+SOURCES = ["kernels_image.hip", "kernels_sgm.hip", "kernels_rows.hip", "static_scratch.hip", "matcher.cpp"]
 ARCH = os.environ.get("R3DG_ARCH", "gfx950")
 
 

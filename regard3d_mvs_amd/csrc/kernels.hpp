@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 
 #include "sgm_types.hpp"
 
@@ -84,6 +85,17 @@ void launch_apply_mask(float* map, std::size_t n, const std::uint8_t* bad, hipSt
 void launch_speckles(float* d, int w, int h, int min_size, float max_diff, std::int32_t* parent, std::int32_t* size,
                      hipStream_t s);
 void launch_median3x3(const float* in, int w, int h, float* out, hipStream_t s);
+
+// Fixed scratch of the entry points that allocate nothing (static_scratch.hip):
+// a device array of 2 * kStaticSpecklePixels int32 per GPU (32 MiB), the
+// speckle filter's parents and component sizes.  static_scratch_init looks up
+// the array's device address of `device` (which must be current) once;
+// static_scratch returns it (nullptr before a successful init).  Calls that
+// use the array on one GPU take static_scratch_mutex(device).
+constexpr std::int64_t kStaticSpecklePixels = std::int64_t{1} << 22;
+hipError_t static_scratch_init(int device);
+std::int32_t* static_scratch(int device);
+std::mutex& static_scratch_mutex(int device);
 
 // ranges_from_coarse (stereo_matcher.cpp:1124-1267).  scratch needs
 // ranges_scratch_floats(wc, hc) floats.

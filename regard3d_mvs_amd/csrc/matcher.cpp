@@ -1709,6 +1709,10 @@ int32_t r3dg_ctx_create(int32_t device, uint64_t hbm_budget, r3dg_ctx** out) {
     auto c = std::make_unique<Context>(budget);
     c->device = device;
     hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+    // Device address of this GPU's fixed scratch array (entry points that
+    // allocate nothing), looked up once per device with the device current.
+    // A failed lookup is reported by those entry points, not here.
+    (void)static_scratch_init(device);
     *out = reinterpret_cast<r3dg_ctx*>(c.release());
   });
 }
@@ -2584,6 +2588,37 @@ int32_t r3dg_remove_speckles(r3dg_ctx* h, r3dg_image disparity, int32_t min_size
     speckles(c, work, min_size, max_diff);
     output_img(c, disparity, work.p, work.w, work.h, work.s, "disparity");
     sync(c);
+  });
+}
+
+static_assert(R3DG_STATIC_SPECKLE_PIXELS == kStaticSpecklePixels, "r3dg.h and the scratch array agree");
+
+int32_t r3dg_remove_speckles_static(r3dg_ctx* h, r3dg_image disparity, int32_t min_size, float max_diff) {
+  return with_ctx(h, [&](Context& c) {
+    if (view_empty(disparity) || min_size <= 1) return;
+    if (disparity.memory != R3DG_DEVICE)
+      fail(R3DG_ERROR_INVALID_ARGUMENT, "remove_speckles_static: the map must be in device memory");
+    if (disparity.format != R3DG_FORMAT_NATIVE)
+      fail(R3DG_ERROR_INVALID_ARGUMENT, "remove_speckles_static: this input takes no format conversion");
+    if (disparity.stride != disparity.width)
+      fail(R3DG_ERROR_INVALID_ARGUMENT, "remove_speckles_static: the map must be contiguous (stride == width)");
+    const std::int64_t n = std::int64_t(disparity.width) * std::int64_t(disparity.height);
+    if (disparity.width <= 0 || disparity.height <= 0 || n > kStaticSpecklePixels)
+      fail(R3DG_ERROR_INVALID_ARGUMENT, "remove_speckles_static: " + std::to_string(disparity.width) + "x" +
+                                            std::to_string(disparity.height) + " exceeds the fixed scratch of " +
+                                            std::to_string(kStaticSpecklePixels) + " pixels");
+    std::int32_t* scr = static_scratch(c.device);
+    if (scr == nullptr)
+      fail(R3DG_ERROR_DEVICE, std::string("remove_speckles_static: hipGetSymbolAddress(scratch) failed: ") +
+                                  hipGetErrorString(static_scratch_init(c.device)));
+    // Every call on this GPU shares the array: one at a time, finished
+    // before the next starts.  The kernels write every parent and size they
+    // read, so nothing is cleared first.
+    const std::lock_guard<std::mutex> use(static_scratch_mutex(c.device));
+    launch_speckles(static_cast<float*>(disparity.data), disparity.width, disparity.height, min_size, max_diff, scr,
+                    scr + kStaticSpecklePixels, c.stream);
+    check_launch();
+    hip_check(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
   });
 }
 
