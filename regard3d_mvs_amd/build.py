@@ -88,6 +88,26 @@ def _build_api() -> None:
     os.replace(API_TEST + ".tmp", API_TEST)
 
 
+def build_sanitized() -> str:
+    """Host AddressSanitizer + UndefinedBehaviorSanitizer builds of
+    libr3d_stereo.so and the C++ API test program (the HIP library itself is
+    the regular one: no GPU sanitizer is used).  Returns the test program."""
+    inc = os.path.join(ROOT, "include")
+    san = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined", "-fno-sanitize-recover=all"]
+    cxx = ["g++", "-std=c++20", "-fPIC", "-Wall", "-Wextra", "-I", inc] + san
+    tdir = os.path.dirname(API_TEST)
+    os.makedirs(tdir, exist_ok=True)
+    api = os.path.join(tdir, "libr3d_stereo_asan.so")
+    _run(cxx + ["-shared", "-o", api, os.path.join(CSRC, "api", "r3d_stereo.cpp"), "-L", HERE, "-lr3dg",
+                f"-Wl,-rpath,{HERE}"])
+    cpp = os.path.join(ROOT, "tests", "cpp")
+    out = API_TEST + "_asan"
+    _run(cxx + ["-ffp-contract=off", "-pthread", "-o", out, os.path.join(cpp, "test_stereo_api.cpp"),
+                os.path.join(cpp, "reference_shim.cpp"), os.path.join(ROOT, "oracle", "oracle.cpp"),
+                api, "-L", HERE, "-lr3dg", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN"])
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
     sys.exit(0)

@@ -1291,10 +1291,13 @@ class Server {
         cv_.wait(lock, [&] { return stop_ || !pending_.empty() || !active.empty() || !view_ops_.empty(); });
         if (stop_ && pending_.empty() && active.empty() && view_ops_.empty()) return;
         ops.swap(view_ops_);
-        // Admission by estimated footprint; a pair that fits nowhere runs alone.
+        // Admission by estimated footprint against the budget left beside
+        // the cached views (and below the context's own device budget); a
+        // pair that fits nowhere runs alone.
+        const std::uint64_t cap = admission_cap();
         while (!pending_.empty()) {
           Request* r = reqs_.at(pending_.front()).get();
-          if ((!active.empty() || !admit.empty()) && in_use_ + r->footprint > budget_) break;
+          if ((!active.empty() || !admit.empty()) && in_use_ + r->footprint > cap) break;
           in_use_ += r->footprint;
           admit.push_back(r);
           pending_.pop_front();
@@ -1322,7 +1325,13 @@ class Server {
           vc[op.id] = std::move(v);
         });
       }
+      if (!ops.empty()) update_view_bytes();
+      std::vector<Request*> retry;  // admitted while memory was short: back to the queue
       for (Request* r : admit) {
+        if (!retry.empty()) {
+          retry.push_back(r);
+          continue;
+        }
         const int32_t st = guarded(&c_, [&] {
           r3dg_pair& q = *r->pair;
           std::memset(&q.info, 0, sizeof(q.info));
@@ -1338,10 +1347,13 @@ class Server {
         });
         if (st == R3DG_OK) {
           active.push_back(r);
+        } else if (st == R3DG_ERROR_BUDGET && !active.empty()) {
+          retry.push_back(r);
         } else {
           complete(r, st);
         }
       }
+      requeue(retry);
       if (active.empty()) continue;
       std::vector<PairRun*> runs;
       for (Request* r : active) runs.push_back(&r->run);
@@ -1363,6 +1375,15 @@ class Server {
         }
         sync(c_);
       });
+      if (st == R3DG_ERROR_BUDGET && active.size() > 1) {
+        // Device memory ran out with several pairs in flight: start them
+        // again later, in their order, and admit less from now on.
+        budget_scale_ *= 0.75;
+        requeue(active);
+        active.clear();
+        update_view_bytes();
+        continue;
+      }
       std::vector<Request*> still;
       for (Request* r : active) {
         if (st != R3DG_OK) {
@@ -1374,12 +1395,49 @@ class Server {
         }
       }
       active.swap(still);
+      update_view_bytes();
     }
+  }
+
+  // Device bytes the admission may promise to requests: the loop's budget,
+  // at most the context's own, times the scale that memory shortages
+  // lowered, minus the cached views.
+  std::uint64_t admission_cap() const {
+    const double cap = static_cast<double>(std::min(budget_, c_.arena.budget())) * budget_scale_;
+    const double views = static_cast<double>(view_bytes_.load(std::memory_order_relaxed));
+    return cap > views ? static_cast<std::uint64_t>(cap - views) : 0;
+  }
+  void update_view_bytes() {
+    std::uint64_t b = 0;
+    if (c_.views)
+      for (const auto& [id, v] : view_cache(c_)) b += v->bytes();
+    view_bytes_.store(b, std::memory_order_relaxed);
+  }
+  // Requests back to the front of the queue, their state released.
+  void requeue(std::vector<Request*>& rs) {
+    if (rs.empty()) return;
+    const std::lock_guard<std::mutex> lock(m_);
+    for (auto it = rs.rbegin(); it != rs.rend(); ++it) {
+      Request* r = *it;
+      r->run = PairRun{};
+      r->lv.reset();
+      r->rv.reset();
+      in_use_ -= r->footprint;
+      for (const auto& [t, q] : reqs_) {
+        if (q.get() == r) {
+          pending_.push_front(t);
+          break;
+        }
+      }
+    }
+    rs.clear();
   }
 
   Context& c_;
   const r3dg_options o_;
   const std::uint64_t budget_;
+  double budget_scale_ = 1.0;                      // loop thread only
+  std::atomic<std::uint64_t> view_bytes_{0};       // cached views, updated by the loop
   std::mutex m_;
   std::condition_variable cv_, done_cv_;
   std::deque<std::uint64_t> pending_;

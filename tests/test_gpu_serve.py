@@ -105,3 +105,23 @@ def test_serve_errors():
                       right=reqs[2]["right"])
         assert m.wait(t2) is not None
         m.serve_stop()
+
+
+def test_serve_under_memory_pressure():
+    """A context budget far below what the admitted pairs need together: the
+    loop restarts the pairs in flight when a step runs out of device memory
+    and admits fewer; every pair still completes, bit-identical to alone."""
+    from regard3d_mvs_amd.native import Matcher
+    pairs = [synth.make_pair(480, 320, 64, 950 + i) for i in range(6)]
+    o = StereoOptions()
+    with Matcher(0, hbm_budget=160 << 20) as m:
+        alone = [m.match(p.left, p.right, None, None, p.dmin, p.nd, o) for p in pairs]
+        m.serve_start(o, hbm_budget=64 << 30)  # admission alone would take all six at once
+        try:
+            tickets = [m.submit(p.left.shape, p.dmin, p.nd, left=p.left, right=p.right) for p in pairs]
+            got = [m.wait(t) for t in tickets]
+        finally:
+            m.serve_stop()
+    for g, a in zip(got, alone):
+        same_bits(g.disparity, a.disparity)
+        same_bits(g.confidence, a.confidence)
