@@ -382,16 +382,23 @@ def bench_serve(args) -> int:
         th = threading.Thread(target=client)
         th.start()
         lat, exact, done = [], True, 0
+        seen = None
         for i in range(len(reqs)):
             while tickets[i] is None:
                 time.sleep(0.0005)
-            g = m.wait(tickets[i])
+            while (g := m.wait(tickets[i], 5.0)) is None:
+                # A line only when the loop advanced (steps or completions).
+                st = m.server_stats()
+                if (st["steps"], st["completed"]) != seen:
+                    seen = (st["steps"], st["completed"])
+                    log(f"bench: rate {rate}: waiting for request {i}; loop {json.dumps(st)}")
             lat.append(late[i] + g.seconds)  # scheduled arrival to completion
             exact = exact and digest(g.disparity) == plain[i]
             done = time.perf_counter()
             if (i + 1) % 250 == 0:
                 log(f"bench: rate {rate}: {i + 1}/{len(reqs)} served")
         th.join()
+        loop_stats = m.server_stats()
         m.serve_stop()
         for v in put:
             m.view_evict(v)
@@ -401,7 +408,7 @@ def bench_serve(args) -> int:
                "latency_ms": {q: 1e3 * percentile(lat, int(q[1:])) for q in ("p50", "p95", "p99")},
                "plain_fifo_latency_ms": {q: 1e3 * percentile(fl, int(q[1:])) for q in ("p50", "p95", "p99")},
                "plain_pairs_per_s": len(reqs) / float(np.max(arrivals + fl)),
-               "bit_exact_vs_plain": exact}
+               "bit_exact_vs_plain": exact, "loop": loop_stats}
         log(f"bench: rate {rate}: {json.dumps(res)}")
         results.append(res)
     out = {"metric": "served pairs/s and p50/p95/p99 latency vs plain", "unit": "pairs/s", "n_gpus": 1,

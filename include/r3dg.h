@@ -255,6 +255,31 @@ int32_t r3dg_match_batch(r3dg_ctx* ctx, r3dg_pair* pairs, int32_t count, const r
 int32_t r3dg_server_start(r3dg_ctx* ctx, const r3dg_options* options, uint64_t hbm_budget);
 /* Finishes the submitted pairs, then stops the loop. */
 int32_t r3dg_server_stop(r3dg_ctx* ctx);
+/* Counters of the running loop (since r3dg_server_start).  A pair whose
+ * start or step runs out of device memory while other pairs are in flight is
+ * put back at the head of the queue (requeued); the loop then admits nothing
+ * new until a pair completes and lowers its admission budget by a quarter
+ * (admission_scale) after a step that ran out.  Running out is the context's
+ * budget (R3DG_ERROR_BUDGET) or a failed device allocation
+ * (R3DG_ERROR_OUT_OF_MEMORY); a pair alone in flight fails with it. */
+typedef struct r3dg_server_stats {
+  uint64_t submitted;        /* tickets issued */
+  uint64_t completed;        /* pairs finished, with any status */
+  uint64_t failed;           /* of those, finished with an error */
+  uint64_t steps;            /* loop steps (one level of every admitted pair) */
+  uint64_t requeued;         /* pairs put back after running out of memory */
+  uint64_t pending;          /* waiting for admission now */
+  uint64_t active;           /* admitted and not finished now */
+  uint64_t in_use;           /* estimated device bytes of the admitted pairs */
+  uint64_t view_bytes;       /* device bytes of the cached views */
+  double admission_scale;    /* factor on the admission budget, 1 at start */
+} r3dg_server_stats;
+int32_t r3dg_server_stats_get(r3dg_ctx* ctx, r3dg_server_stats* out);
+/* Test hook, no counterpart in the reference: in the next loop step with at
+ * least two pairs in flight, the device allocation after `after` more fails
+ * once with R3DG_ERROR_OUT_OF_MEMORY (dropped if the step makes fewer);
+ * negative disarms it.  Forces the requeue path from the host. */
+int32_t r3dg_server_inject_step_failure(r3dg_ctx* ctx, int64_t after);
 /* left_view / right_view: ids given to r3dg_view_put, used in place of the
  * pair's left + left_valid (right + right_valid) images when non-zero. */
 int32_t r3dg_submit(r3dg_ctx* ctx, r3dg_pair* pair, uint64_t left_view, uint64_t right_view, uint64_t* ticket);

@@ -202,6 +202,15 @@ class CMatchDebug(ctypes.Structure):
     _fields_ = [("labels", CImage), ("lr_mask", CImage)]
 
 
+class CServerStats(ctypes.Structure):
+    """r3dg_server_stats: counters of the running request loop."""
+    _fields_ = [(k, ctypes.c_uint64) for k in ("submitted", "completed", "failed", "steps", "requeued", "pending",
+                                                "active", "in_use", "view_bytes")] + [("admission_scale", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class CPair(ctypes.Structure):
     """r3dg_pair: one pair of r3dg_match_batch."""
     _fields_ = [("left", CImage), ("right", CImage), ("left_valid", CImage), ("right_valid", CImage),

@@ -1147,15 +1147,30 @@ ViewCache& view_cache(Context& c) {
 // costs, 3 bytes per cell) with about 48 labels per pixel when ranges come
 // from a coarser level.  Volumes beyond the memory left are chunked anyway.
 // The part that cannot be chunked: images, pyramids, Census and maps.
+// Sizes of a pair for its memory estimates.  The disparity output has the
+// left image's size also where the left image comes from a cached view (the
+// pair's own left image is then empty); a right image from a view is taken
+// as wide as the left.
+struct PairSize {
+  std::uint64_t wl, wr, h;
+};
+PairSize pair_size(const r3dg_pair& q) {
+  const std::uint64_t wl = std::uint64_t(std::max(q.disparity.width, 0));
+  const std::uint64_t h = std::uint64_t(std::max(q.disparity.height, 0));
+  const std::uint64_t wr = view_empty(q.right) ? wl : std::uint64_t(std::max(q.right.width, 0));
+  return {wl, wr, h};
+}
+
 std::uint64_t pair_fixed_bytes(const r3dg_pair& q) {
-  const std::uint64_t wl = std::uint64_t(std::max(q.left.width, 0)), wr = std::uint64_t(std::max(q.right.width, 0));
-  return 96 * (wl + wr) * std::uint64_t(std::max(q.left.height, 0));
+  const PairSize z = pair_size(q);
+  return 96 * (z.wl + z.wr) * z.h;
 }
 
 std::uint64_t pair_footprint(const r3dg_pair& q, const r3dg_options& o) {
-  const std::uint64_t wl = std::uint64_t(std::max(q.left.width, 0));
-  const std::uint64_t h = std::uint64_t(std::max(q.left.height, 0));
-  const int levels = o.levels >= 0 ? o.levels : automatic_levels(q.left.width, q.left.height, q.num_disparities);
+  const PairSize z = pair_size(q);
+  const std::uint64_t wl = z.wl, h = z.h;
+  const int levels = o.levels >= 0 ? o.levels
+                                   : automatic_levels(static_cast<int>(wl), static_cast<int>(h), q.num_disparities);
   const std::uint64_t labels = levels > 0 ? 48 : std::uint64_t(align16(std::max(q.num_disparities, 1)));
   return pair_fixed_bytes(q) + 3 * 12 * wl * h * labels / 5;
 }
@@ -1235,6 +1250,24 @@ class Server {
     return t;
   }
 
+  void inject_step_failure(std::int64_t after) { step_fail_after_.store(after < 0 ? -1 : after); }
+
+  r3dg_server_stats stats() {
+    const std::lock_guard<std::mutex> lock(m_);
+    r3dg_server_stats s{};
+    s.submitted = next_ - 1;
+    s.completed = completed_;
+    s.failed = failed_;
+    s.steps = steps_.load(std::memory_order_relaxed);
+    s.requeued = requeued_;
+    s.pending = pending_.size();
+    s.active = active_n_.load(std::memory_order_relaxed);
+    s.in_use = in_use_;
+    s.view_bytes = view_bytes_.load(std::memory_order_relaxed);
+    s.admission_scale = scale_.load(std::memory_order_relaxed);
+    return s;
+  }
+
   // R3DG_PENDING, or the finished pair's status (the ticket is released).
   int32_t wait(std::uint64_t t, double timeout) {
     std::unique_lock<std::mutex> lock(m_);
@@ -1264,7 +1297,10 @@ class Server {
       r->pair->status = status;
       r->status = status;
       in_use_ -= r->footprint;
+      ++completed_;
+      if (status != R3DG_OK) ++failed_;
     }
+    hold_ = false;  // memory was returned: admission may resume
     done_cv_.notify_all();
   }
 
@@ -1294,8 +1330,11 @@ class Server {
         // Admission by estimated footprint against the budget left beside
         // the cached views (and below the context's own device budget); a
         // pair that fits nowhere runs alone.
+        // After a memory shortage nothing new is admitted until a pair in
+        // flight completes: admitting again at once would fail again, after
+        // freeing the arena's cached blocks, at every step.
         const std::uint64_t cap = admission_cap();
-        while (!pending_.empty()) {
+        while (!pending_.empty() && !(hold_ && !active.empty())) {
           Request* r = reqs_.at(pending_.front()).get();
           if ((!active.empty() || !admit.empty()) && in_use_ + r->footprint > cap) break;
           in_use_ += r->footprint;
@@ -1347,16 +1386,26 @@ class Server {
         });
         if (st == R3DG_OK) {
           active.push_back(r);
-        } else if (st == R3DG_ERROR_BUDGET && !active.empty()) {
+        } else if (shortage(st) && !active.empty()) {
           retry.push_back(r);
         } else {
           complete(r, st);
         }
       }
+      if (!retry.empty()) hold_ = true;
       requeue(retry);
+      active_n_.store(active.size(), std::memory_order_relaxed);
       if (active.empty()) continue;
       std::vector<PairRun*> runs;
       for (Request* r : active) runs.push_back(&r->run);
+      bool armed = false;  // test hook: a failure inside a step of several pairs
+      if (active.size() >= 2) {
+        const std::int64_t k = step_fail_after_.exchange(-1);
+        if (k >= 0) {
+          c_.arena.inject_failure(k);
+          armed = true;
+        }
+      }
       const int32_t st = guarded(&c_, [&] {
         match_step(c_, runs, o_);
         for (Request* r : active) {
@@ -1375,12 +1424,16 @@ class Server {
         }
         sync(c_);
       });
-      if (st == R3DG_ERROR_BUDGET && active.size() > 1) {
+      if (armed) c_.arena.inject_failure(-1);  // one the step did not reach is dropped
+      steps_.fetch_add(1, std::memory_order_relaxed);
+      if (shortage(st) && active.size() > 1) {
         // Device memory ran out with several pairs in flight: start them
         // again later, in their order, and admit less from now on.
         budget_scale_ *= 0.75;
+        scale_.store(budget_scale_, std::memory_order_relaxed);
         requeue(active);
         active.clear();
+        active_n_.store(0, std::memory_order_relaxed);
         update_view_bytes();
         continue;
       }
@@ -1395,9 +1448,13 @@ class Server {
         }
       }
       active.swap(still);
+      active_n_.store(active.size(), std::memory_order_relaxed);
       update_view_bytes();
     }
   }
+
+  // Out of device memory: the arena's budget or the device itself.
+  static bool shortage(int32_t st) { return st == R3DG_ERROR_BUDGET || st == R3DG_ERROR_OUT_OF_MEMORY; }
 
   // Device bytes the admission may promise to requests: the loop's budget,
   // at most the context's own, times the scale that memory shortages
@@ -1423,6 +1480,7 @@ class Server {
       r->lv.reset();
       r->rv.reset();
       in_use_ -= r->footprint;
+      ++requeued_;
       for (const auto& [t, q] : reqs_) {
         if (q.get() == r) {
           pending_.push_front(t);
@@ -1437,7 +1495,12 @@ class Server {
   const r3dg_options o_;
   const std::uint64_t budget_;
   double budget_scale_ = 1.0;                      // loop thread only
+  bool hold_ = false;                              // loop thread only: no admission until a completion
   std::atomic<std::uint64_t> view_bytes_{0};       // cached views, updated by the loop
+  std::atomic<double> scale_{1.0};                 // budget_scale_ for the counters
+  std::atomic<std::uint64_t> steps_{0}, active_n_{0};
+  std::atomic<std::int64_t> step_fail_after_{-1};
+  std::uint64_t completed_ = 0, failed_ = 0, requeued_ = 0;  // under m_
   std::mutex m_;
   std::condition_variable cv_, done_cv_;
   std::deque<std::uint64_t> pending_;
@@ -2515,6 +2578,26 @@ int32_t r3dg_server_stop(r3dg_ctx* h) {
   // The loop takes the context lock for its steps: stop it without holding it.
   if (srv) static_cast<Server*>(srv.get())->stop();
   return R3DG_OK;
+}
+
+int32_t r3dg_server_inject_step_failure(r3dg_ctx* h, int64_t after) {
+  if (h == nullptr) return with_ctx(nullptr, [](Context&) {});
+  auto* c = reinterpret_cast<Context*>(h);
+  return guarded(nullptr, [&] {
+    const std::shared_ptr<Server> srv = running_server(*c);
+    if (!srv) fail(R3DG_ERROR_INVALID_ARGUMENT, "serve: no request loop (r3dg_server_start)");
+    srv->inject_step_failure(after);
+  });
+}
+
+int32_t r3dg_server_stats_get(r3dg_ctx* h, r3dg_server_stats* out) {
+  if (h == nullptr || out == nullptr) return with_ctx(nullptr, [](Context&) {});
+  auto* c = reinterpret_cast<Context*>(h);
+  return guarded(nullptr, [&] {
+    const std::shared_ptr<Server> srv = running_server(*c);
+    if (!srv) fail(R3DG_ERROR_INVALID_ARGUMENT, "serve: no request loop (r3dg_server_start)");
+    *out = srv->stats();
+  });
 }
 
 int32_t r3dg_submit(r3dg_ctx* h, r3dg_pair* pair, uint64_t left_view, uint64_t right_view, uint64_t* ticket) {

@@ -12,8 +12,8 @@ import re
 
 import numpy as np
 
-from ._abi import (R3DG_DEVICE, R3DG_FORMAT_U8X257, R3DG_HOST, STAGES, CImage, CMatchDebug, CMatchInfo, CPair, DisparityMap, Status, StereoOptions,
-                   align16)
+from ._abi import (R3DG_DEVICE, R3DG_FORMAT_U8X257, R3DG_HOST, STAGES, CImage, CMatchDebug, CMatchInfo, CPair,
+                   CServerStats, DisparityMap, Status, StereoOptions, align16)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libr3dg.so")
@@ -423,6 +423,17 @@ class Matcher:
 
     def serve_stop(self):
         self._check(lib().r3dg_server_stop(self._ctx))
+
+    def inject_step_failure(self, after):
+        """Test hook: in the next loop step with two or more pairs in flight,
+        the device allocation after `after` more fails once."""
+        self._check(lib().r3dg_server_inject_step_failure(self._ctx, ctypes.c_int64(int(after))))
+
+    def server_stats(self) -> dict:
+        """Counters of the running request loop (r3dg_server_stats_get)."""
+        st = CServerStats()
+        self._check(lib().r3dg_server_stats_get(self._ctx, ctypes.byref(st)))
+        return st.as_dict()
 
     def view_put(self, view_id, image, valid=None):
         v, k1 = _image16(image)

@@ -91,6 +91,37 @@ def test_serve_cached_views_and_threads():
         same_bits(g.confidence, alone[k % 3].confidence)
 
 
+def test_serve_admission_counts_pairs_from_views():
+    """Pairs whose images all come from cached views are admitted by their
+    estimated footprint too: with an admission budget of one byte the loop
+    runs them strictly one at a time, and every map equals the pair alone."""
+    from regard3d_mvs_amd.native import Matcher
+    gens, _ = views_and_requests()
+    o = StereoOptions()
+    with Matcher(0) as m:
+        for i, p in enumerate(gens):
+            m.view_put(10 + i, p.left)
+            m.view_put(20 + i, p.right)
+        alone = [m.match(p.left, p.right, None, None, p.dmin, p.nd, o) for p in gens]
+        m.serve_start(o, hbm_budget=1)
+        try:
+            tickets = [m.submit(gens[k % 3].left.shape, gens[k % 3].dmin, gens[k % 3].nd, left_view=10 + k % 3,
+                                right_view=20 + k % 3) for k in range(6)]
+            most = 0
+            while True:
+                st = m.server_stats()
+                most = max(most, st["active"])
+                if st["completed"] == len(tickets):
+                    break
+                time.sleep(0.0005)
+            got = [m.wait(t) for t in tickets]
+        finally:
+            m.serve_stop()
+    assert most <= 1, most
+    for k, g in enumerate(got):
+        same_bits(g.disparity, alone[k % 3].disparity)
+
+
 def test_serve_errors():
     from regard3d_mvs_amd.native import Matcher, R3dgError
     gens, reqs = views_and_requests()
@@ -108,9 +139,9 @@ def test_serve_errors():
 
 
 def test_serve_under_memory_pressure():
-    """A context budget far below what the admitted pairs need together: the
-    loop restarts the pairs in flight when a step runs out of device memory
-    and admits fewer; every pair still completes, bit-identical to alone."""
+    """A context budget far below what the admitted pairs would need with
+    their volumes whole: the steps chunk the volumes to the budget and every
+    pair completes, bit-identical to alone."""
     from regard3d_mvs_amd.native import Matcher
     pairs = [synth.make_pair(480, 320, 64, 950 + i) for i in range(6)]
     o = StereoOptions()
@@ -120,8 +151,36 @@ def test_serve_under_memory_pressure():
         try:
             tickets = [m.submit(p.left.shape, p.dmin, p.nd, left=p.left, right=p.right) for p in pairs]
             got = [m.wait(t) for t in tickets]
+            st = m.server_stats()
         finally:
             m.serve_stop()
     for g, a in zip(got, alone):
         same_bits(g.disparity, a.disparity)
         same_bits(g.confidence, a.confidence)
+    assert st["submitted"] == st["completed"] == len(pairs) and st["failed"] == 0, st
+    assert st["pending"] == 0 and st["active"] == 0 and st["steps"] >= 2, st
+
+
+@pytest.mark.parametrize("after", [0, 40])
+def test_serve_requeues_pairs_when_a_step_runs_out(after):
+    """A device allocation that fails inside a step of several pairs: the pairs
+    go back to the queue, are admitted again and complete bit-identical to
+    alone; none fails."""
+    from regard3d_mvs_amd.native import Matcher
+    pairs = [synth.make_pair(480, 320, 64, 970 + i) for i in range(6)]
+    o = StereoOptions()
+    with Matcher(0) as m:
+        alone = [m.match(p.left, p.right, None, None, p.dmin, p.nd, o) for p in pairs]
+        m.serve_start(o)
+        try:
+            m.inject_step_failure(after)
+            tickets = [m.submit(p.left.shape, p.dmin, p.nd, left=p.left, right=p.right) for p in pairs]
+            got = [m.wait(t) for t in tickets]
+            st = m.server_stats()
+        finally:
+            m.serve_stop()
+    for g, a in zip(got, alone):
+        same_bits(g.disparity, a.disparity)
+        same_bits(g.confidence, a.confidence)
+    assert st["completed"] == len(pairs) and st["failed"] == 0, st
+    assert st["requeued"] >= 2 and st["admission_scale"] == 0.75, st
