@@ -632,9 +632,12 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
   };
   // Segment 1 only when a scanline of the wave needs it at that step (its
   // pixel has more than 32 labels); the step tests the same condition.
+  // The label count / 16 is the top byte of meta: more than 32 labels is
+  // meta > 0x02FFFFFF, never for idle groups.
+  const std::uint32_t seg1_meta = grp_active ? 0x02FFFFFFU : 0xFFFFFFFFU;
   auto load_segs = [&](const StepIn& u, StepSegs& d) {
     load_seg4(u, d.a);
-    if (__builtin_amdgcn_ballot_w64((grp_active ? step_num(u) : 0) > 32) != 0) load_seg8(u, 1, d.b);
+    if (__builtin_amdgcn_ballot_w64(u.meta > seg1_meta) != 0) load_seg8(u, 1, d.b);
   };
 
   int prev_lo = 0, prev_num = 0, flip = 0;
@@ -720,11 +723,17 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
       for (int k = 0; k < P / 2 + 2; ++k) W[k] = wp[k];
     };
     // Stores of a lane's new path costs (next step's row) and S; running min.
-    auto emit = [&](auto Pc, int j0, const std::uint32_t* L, const std::uint32_t* sv) {
+    // kLow: segments 0 and 1, whose labels (below 96) lie inside the LDS row
+    // of every lane.  Their row store needs no condition: a lane past its
+    // pixel's labels, or of a finished scanline, writes only positions the
+    // next step does not read (the sentinels after the labels are written
+    // later in the step, and reads stop inside them), and a wide pixel's row
+    // is read from scratch.
+    auto emit = [&](auto Pc, auto kLow, int j0, const std::uint32_t* L, const std::uint32_t* sv) {
       constexpr int P = decltype(Pc)::value;
       const bool on = act && j0 < num;
       const int w0 = (kPad + j0) >> 1;
-      if (on && !cur_wide) {
+      if (decltype(kLow)::value || (on && !cur_wide)) {
 #pragma unroll
         for (int k = 0; k < P / 2; k += 2) *reinterpret_cast<u32x2*>(cur + w0 + k) = u32x2{L[k], L[k + 1]};
       }
@@ -773,14 +782,14 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
       std::uint32_t W[6], sh, L[4];
       window(P8{}, j0, W, sh);
       recur(P8{}, W, sh, d.c, L);
-      emit(P8{}, j0, L, d.s);
+      emit(P8{}, std::true_type{}, j0, L, d.s);
     };
     auto seg4 = [&](const SegIn<4>& d) {
       const int j0 = seg_first<4>(0, l8);
       std::uint32_t W[4], sh, L[2];
       window(P4{}, j0, W, sh);
       recur(P4{}, W, sh, d.c, L);
-      emit(P4{}, j0, L, d.s);
+      emit(P4{}, std::true_type{}, j0, L, d.s);
     };
     // Segments from 2 on (pixels of more than 96 labels): costs, S and the
     // scratch window of every lane (used where prev_wide) loaded one segment
@@ -806,7 +815,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
         for (int k = 0; k < 6; ++k) W[k] = prev_wide ? x.G[k] : W[k];
         sh = prev_wide ? gsh : sh;
         recur(P8{}, W, sh, x.d.c, L);
-        emit(P8{}, j0, L, x.d.s);
+        emit(P8{}, std::false_type{}, j0, L, x.d.s);
       };
       load(2, A);
       for (int s = 2; s < segmax; s += 2) {
@@ -821,14 +830,12 @@ __global__ __launch_bounds__(64 * kWaves) void k_path(
     seg4(v.a);
     if (any_seg1) seg8(1, v.b);
     if (any_seg2) wide();
-    // Sentinels after the labels.
-    if (act && l8 < kPad / 2) {
+    // Sentinels after the labels (in the LDS row also for finished
+    // scanlines, whose rows are not read again).
+    if (l8 < kPad / 2) {
       const int w = ((kPad + num) >> 1) + l8;
-      if (cur_wide) {
-        scur[w] = 0xFFFFFFFFU;
-      } else {
-        cur[w] = 0xFFFFFFFFU;
-      }
+      if (!cur_wide) cur[w] = 0xFFFFFFFFU;
+      if (any_cur_wide && act && cur_wide) scur[w] = 0xFFFFFFFFU;
     }
     prev_min = min8(min(m & 0xFFFFU, m >> 16));
     prev_lo = lo;

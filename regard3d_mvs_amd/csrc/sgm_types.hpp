@@ -26,7 +26,7 @@
 namespace r3dg {
 
 constexpr int kLabelAlign = 16;
-constexpr int kMaxLabels = 4000;  // per pixel: two LDS rows per scanline must fit 64 KiB
+constexpr int kMaxLabels = 4000;  // per pixel and level: the path kernel keeps label count / 16 in 8 bits
 
 __host__ __device__ constexpr std::uint64_t pix_record(std::uint32_t off, int lo, int num) {
   return std::uint64_t(off) | (std::uint64_t(std::uint16_t(lo)) << 32) | (std::uint64_t(std::uint16_t(num)) << 48);
