@@ -929,6 +929,27 @@ __device__ __forceinline__ std::uint32_t key_min(const uint4& q, int k0, std::ui
   return m;
 }
 
+// Inserts the keys (value << 16 | label) of labels k0..k0+7 into the sorted
+// four smallest keys t0 <= t1 <= t2 <= t3.
+__device__ __forceinline__ void top4(const uint4& q, int k0, std::uint32_t& t0, std::uint32_t& t1,
+                                     std::uint32_t& t2, std::uint32_t& t3) {
+  const std::uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    std::uint32_t c = (((w[e >> 1] >> ((e & 1) * 16)) & 0xFFFFU) << 16) | static_cast<std::uint32_t>(k0 + e);
+    std::uint32_t a = min(t0, c);
+    c = max(t0, c);
+    t0 = a;
+    a = min(t1, c);
+    c = max(t1, c);
+    t1 = a;
+    a = min(t2, c);
+    c = max(t2, c);
+    t2 = a;
+    t3 = min(t3, c);
+  }
+}
+
 // Lowest value of labels k0..k0+7 outside [kb - 1, kb + 1].
 __device__ __forceinline__ std::uint32_t outside_min(const uint4& q, int k0, int kb, std::uint32_t m) {
   const std::uint32_t w[4] = {q.x, q.y, q.z, q.w};
@@ -1054,13 +1075,24 @@ __global__ __launch_bounds__(256) void k_select(const JobDesc* __restrict__ jobs
       std::uint32_t m = 0xFFFFFFFFU;
       if (k0 < n) m = key_min(g0, k0, m);
       if (k1 < n) m = key_min(g1, k1, m);
-      for (int k = 64 + 8 * q; k < n; k += 32) m = key_min(ld8(s + k), k, m);
-      m = quad_min(m);
+      // Labels from 64 on are read once: the lane keeps its four smallest
+      // keys among them (t0 <= t1 <= t2 <= t3).  At most three labels are
+      // excluded from the second minimum (kb - 1, kb, kb + 1), so the first
+      // of these keys outside that interval holds the lane's second minimum
+      // over its labels from 64 on.
+      std::uint32_t t0 = 0xFFFFFFFFU, t1 = t0, t2 = t0, t3 = t0;
+      for (int k = 64 + 8 * q; k < n; k += 32) top4(ld8(s + k), k, t0, t1, t2, t3);
+      m = quad_min(min(m, t0));
       const int kb = static_cast<int>(m & 0xFFFFU);
       std::uint32_t second = 0xFFFFU;
       if (k0 < n) second = outside_min(g0, k0, kb, second);
       if (k1 < n) second = outside_min(g1, k1, kb, second);
-      for (int k = 64 + 8 * q; k < n; k += 32) second = outside_min(ld8(s + k), k, kb, second);
+      auto outside = [&](std::uint32_t t) {
+        const int k = static_cast<int>(t & 0xFFFFU);
+        return k + 1 < kb || k > kb + 1;
+      };
+      const std::uint32_t tw = outside(t0) ? t0 : (outside(t1) ? t1 : (outside(t2) ? t2 : t3));
+      second = min(second, tw >> 16);
       second = quad_min(second);
       // Costs at kb - 1 and kb + 1 from the registers of the quad when they
       // lie in the first 64 labels.
