@@ -283,7 +283,8 @@ __device__ __forceinline__ void scan_start(int s, int w, int h, int& x, int& y, 
 // chunk-to-pixel map, the partner Census window), so the chunk loop touches
 // global memory only to store; wider rows search their pixel records in
 // global memory.
-__global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, const TileDesc* __restrict__ tiles,
+constexpr int kCostThreads = 256;
+__global__ __launch_bounds__(kCostThreads) void k_cost(const JobDesc* __restrict__ jobs, const TileDesc* __restrict__ tiles,
                                               const std::uint64_t* __restrict__ row_prefix, int ntiles,
                                               const std::uint64_t* __restrict__ bcorr,
                                               const std::uint64_t* __restrict__ pix, std::uint8_t* __restrict__ C) {
@@ -320,20 +321,49 @@ __global__ __launch_bounds__(256) void k_cost(const JobDesc* __restrict__ jobs, 
   const bool windowed = c1 >= c0 && c1 - c0 < kWin;
   const bool staged = T.w <= kPx && T.nbands <= kBands;
   const std::uint64_t last = rec[T.w - 1];
-  if (windowed) {
-    for (int k = threadIdx.x; k <= c1 - c0; k += blockDim.x) swin[k] = cr[pcol(c0 + k)];
+  // Staging: every global load of a thread is issued before the first LDS
+  // store, so that the block waits for one memory latency, not one per
+  // element (the launch has kCostThreads threads).
+  constexpr int kWinIt = (kWin + kCostThreads - 1) / kCostThreads;
+  constexpr int kPxIt = (kPx + kCostThreads - 1) / kCostThreads;
+  static_assert(kBands <= kCostThreads, "one band correction per thread");
+  const int tid = static_cast<int>(threadIdx.x);
+  std::uint64_t wv[kWinIt], pv[kPxIt], cv[kPxIt], bv = 0;
+#pragma unroll
+  for (int i = 0; i < kWinIt; ++i) {
+    const int k = tid + kCostThreads * i;
+    wv[i] = windowed && k <= c1 - c0 ? cr[pcol(c0 + k)] : 0;
   }
   if (staged) {
-    for (int x = threadIdx.x; x < T.w; x += blockDim.x) {
-      const std::uint64_t pr = rec[x];
-      const std::uint32_t off = pix_off(pr);
-      soff[x] = off;
-      slo[x] = static_cast<std::uint16_t>(pix_lo(pr));
-      scl[x] = cl[bcol(T.x0 + x)];
-      const std::uint32_t e = min((off + static_cast<std::uint32_t>(pix_num(pr))) >> 4, static_cast<std::uint32_t>(kMap));
-      for (std::uint32_t k = off >> 4; k < e; ++k) smap[k] = static_cast<std::uint16_t>(x);
+#pragma unroll
+    for (int i = 0; i < kPxIt; ++i) {
+      const int x = tid + kCostThreads * i;
+      pv[i] = x < T.w ? rec[x] : 0;
+      cv[i] = x < T.w ? cl[bcol(T.x0 + x)] : 0;
     }
-    for (int b = threadIdx.x; b < T.nbands; b += blockDim.x) sbc[b] = bc[std::size_t(b) * T.h];
+    if (tid < T.nbands) bv = bc[std::size_t(tid) * T.h];
+  }
+#pragma unroll
+  for (int i = 0; i < kWinIt; ++i) {
+    const int k = tid + kCostThreads * i;
+    if (windowed && k <= c1 - c0) swin[k] = wv[i];
+  }
+  if (staged) {
+#pragma unroll
+    for (int i = 0; i < kPxIt; ++i) {
+      const int x = tid + kCostThreads * i;
+      if (x < T.w) {
+        const std::uint64_t pr = pv[i];
+        const std::uint32_t off = pix_off(pr);
+        soff[x] = off;
+        slo[x] = static_cast<std::uint16_t>(pix_lo(pr));
+        scl[x] = cv[i];
+        const std::uint32_t e =
+            min((off + static_cast<std::uint32_t>(pix_num(pr))) >> 4, static_cast<std::uint32_t>(kMap));
+        for (std::uint32_t k = off >> 4; k < e; ++k) smap[k] = static_cast<std::uint16_t>(x);
+      }
+    }
+    if (tid < T.nbands) sbc[tid] = bv;
   }
   __syncthreads();
   const std::uint32_t chunks = (pix_off(last) + static_cast<std::uint32_t>(pix_num(last))) >> 4;
@@ -1016,10 +1046,22 @@ __global__ __launch_bounds__(256) void k_select(const JobDesc* __restrict__ jobs
   constexpr int kPerPass = 256 / 4;
   const bool staged = T.w <= kStage;
   if (staged) {
-    for (int x = threadIdx.x; x < T.w; x += blockDim.x) {
-      const std::uint64_t rec = pix[px + x];
-      scell[x] = bc[std::size_t(x >> T.band_shift) * T.h] + pix_off(rec);
-      srange[x] = static_cast<std::uint32_t>(rec >> 32);
+    // Every load of a thread before its first LDS store (256 threads).
+    constexpr int kIt = kStage / 256;
+    std::uint64_t rv[kIt], bv[kIt];
+#pragma unroll
+    for (int i = 0; i < kIt; ++i) {
+      const int x = static_cast<int>(threadIdx.x) + 256 * i;
+      rv[i] = x < T.w ? pix[px + x] : 0;
+      bv[i] = x < T.w ? bc[std::size_t(x >> T.band_shift) * T.h] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < kIt; ++i) {
+      const int x = static_cast<int>(threadIdx.x) + 256 * i;
+      if (x < T.w) {
+        scell[x] = bv[i] + pix_off(rv[i]);
+        srange[x] = static_cast<std::uint32_t>(rv[i] >> 32);
+      }
     }
     __syncthreads();
   }
@@ -1301,7 +1343,7 @@ void launch_cost(const JobDesc* jobs, const TileDesc* tiles, const std::uint64_t
                  std::uint64_t total_rows, const std::uint64_t* bcorr, const std::uint64_t* pix, std::uint8_t* C,
                  hipStream_t s) {
   if (total_rows == 0) return;
-  k_cost<<<dim3(static_cast<unsigned>(total_rows)), 256, 0, s>>>(jobs, tiles, row_prefix, ntiles, bcorr, pix, C);
+  k_cost<<<dim3(static_cast<unsigned>(total_rows)), kCostThreads, 0, s>>>(jobs, tiles, row_prefix, ntiles, bcorr, pix, C);
 }
 
 int path_scratch_row(int max_num) { return kPad + (max_num + 15) / 16 * 16 + kPad; }
