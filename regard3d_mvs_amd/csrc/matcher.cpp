@@ -635,12 +635,15 @@ Chunk aggregate_tiles(Context& c, Aggregation& A, int t0, int t1, bool sum_slice
   // (a multiple of 4: launches use whole blocks of 4 waves, one slot each)
   const std::uint64_t resident = path_slots();
   const std::uint64_t slots = (std::min(resident, std::max<std::uint64_t>(max_groups, 1)) + 3) / 4 * 4;
-  // The scans run side by side on 2 to 4 streams (more where their groups
+  // The scans run side by side on 2 or 3 streams (3 where their groups
   // average less than the device holds), each stream summing its paths into a
   // slice of its own, when the slices and their scratch rows fit the budget:
   // a launch's last groups then overlap the next launch on another stream.
+  // Not 4: a process gets 4 hardware queues and 4 such streams were traced
+  // on 3 of them, so that one queue ran 4 launches in turn (measured: the
+  // 1080p pair's aggregation 5.0 ms with 4 streams, 4.7 ms with 3).
   const std::uint64_t mean_groups = std::max<std::uint64_t>((sum_groups + 7) / 8, 1);
-  int ways = static_cast<int>(std::clamp<std::uint64_t>((resident + mean_groups - 1) / mean_groups, 2, 4));
+  int ways = static_cast<int>(std::clamp<std::uint64_t>((resident + mean_groups - 1) / mean_groups, 2, 3));
   const std::uint64_t slice_bytes =
       (ch.cells + kSPad) * sizeof(std::uint16_t) + path_scratch_elems(slots, max_num) * sizeof(std::uint16_t);
   const std::uint64_t free_bytes = c.arena.budget() > c.arena.live() ? c.arena.budget() - c.arena.live() : 0;
