@@ -206,6 +206,35 @@ void upload(Context& c, DBuf<T>& dst, const std::vector<T>& src) {
   hip_check(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
 }
 
+// Several small host arrays uploaded with one copy: part k starts at a
+// 256-byte aligned offset of one device block (one blit on the stream
+// instead of one per array).
+struct UploadPack {
+  std::vector<unsigned char> host;
+  template <typename T>
+  std::size_t add(const std::vector<T>& v) {
+    const std::size_t off = (host.size() + 255) / 256 * 256;
+    host.resize(off + v.size() * sizeof(T));
+    if (!v.empty()) std::memcpy(host.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+  }
+  std::size_t zeros(std::size_t bytes) {
+    const std::size_t off = (host.size() + 255) / 256 * 256;
+    host.resize(off + bytes, 0);
+    return off;
+  }
+};
+// A typed pointer into a packed block (the block owns the memory).
+template <typename T>
+struct DView {
+  T* p = nullptr;
+  [[nodiscard]] T* get() const { return p; }
+};
+template <typename T>
+DView<T> view(const DBuf<unsigned char>& b, std::size_t off) {
+  return DView<T>{reinterpret_cast<T*>(b.get() + off)};
+}
+
 void sync(Context& c) { c.sync(); }
 void check_launch() { hip_check(hipGetLastError(), "kernel launch"); }
 
@@ -408,13 +437,14 @@ int path_scanlines(int p, int w, int h) { return p < 2 ? h : (p < 4 ? w : w + h 
 // and chunks of consecutive tiles share the cost pass, the 8 path launches
 // and the selection launch.
 struct Aggregation {
-  DBuf<JobDesc> djobs;
+  DBuf<unsigned char> meta;  // job descriptors (and the layout pass's tiles and row prefix)
+  DView<JobDesc> djobs;
   std::vector<TileDesc> td;  // cell_base counts over all tiles
   DBuf<TileDesc> dtiles;
   std::vector<Grid> grids;
   std::vector<int> tile_base;
   std::vector<std::uint64_t> tcells;
-  DBuf<std::uint64_t> drow_prefix, bcorr, pix;  // bcorr: band corrections (TileDesc)
+  DBuf<std::uint64_t> bcorr, pix;  // bcorr: band corrections (TileDesc)
   std::uint64_t total_rows = 0, total_px = 0, total_cells = 0, max_tile_cells = 0;
   int max_w = 1, max_D = 16, max_num = 16;
 };
@@ -423,8 +453,9 @@ struct Aggregation {
 // descriptors count cells from its first tile.
 struct Chunk {
   int t0 = 0, t1 = 0;
-  DBuf<TileDesc> dtiles;
-  DBuf<std::uint64_t> drow_prefix;
+  DBuf<unsigned char> meta;  // tiles, row prefix, group prefixes, work-queue counters
+  DView<TileDesc> dtiles;
+  DView<std::uint64_t> drow_prefix;
   std::uint64_t rows = 0, cells = 0, px = 0;
   // Aggregated costs: S plus the slices of path scans that ran side by side
   // on other streams (the sum mod 2^16 of all of them).
@@ -469,8 +500,6 @@ Aggregation layout_jobs(Context& c, std::vector<Job>& jobs, const r3dg_options& 
     d.confidence = J.confidence ? 1 : 0;
     std::memcpy(d.lut, lut.data(), sizeof(d.lut));
   }
-  DBuf<JobDesc>& djobs = A.djobs;
-  upload(c, djobs, jd);
 
   const int ext = o.tile_margin + (o.tile_blend + 1) / 2;
   std::vector<int> tsize(static_cast<std::size_t>(nj), o.tile_size);
@@ -482,13 +511,13 @@ Aggregation layout_jobs(Context& c, std::vector<Job>& jobs, const r3dg_options& 
   std::vector<std::uint64_t> row_prefix;
   std::vector<std::uint64_t>& tcells = A.tcells;
   std::vector<std::uint32_t> tmax;
-  DBuf<TileDesc>& dtiles = A.dtiles;
-  DBuf<std::uint64_t>& drow_prefix = A.drow_prefix;
+  DView<TileDesc> dtiles;
+  DView<std::uint64_t> drow_prefix;
   DBuf<std::uint64_t>& bcorr = A.bcorr;
   DBuf<std::uint64_t> bstart;
   DBuf<std::uint64_t>& pix = A.pix;
-  DBuf<std::uint64_t> row_cells, tile_cells;
-  DBuf<std::uint32_t> row_max, tile_max;
+  DBuf<std::uint64_t> row_cells, tile_counts;  // tile_counts: cells [nt], then the max label counts (uint32)
+  DBuf<std::uint32_t> row_max;
   std::uint64_t& total_rows = A.total_rows;
   std::uint64_t& total_px = A.total_px;
   std::vector<bool> fixed(static_cast<std::size_t>(nj), false);
@@ -526,29 +555,35 @@ Aggregation layout_jobs(Context& c, std::vector<Job>& jobs, const r3dg_options& 
     }
     tile_base[std::size_t(nj)] = static_cast<int>(td.size());
     const int nt = static_cast<int>(td.size());
-    upload(c, dtiles, td);
-    upload(c, drow_prefix, row_prefix);
+    {
+      UploadPack pk;
+      const std::size_t oj = pk.add(jd), ot = pk.add(td), orp = pk.add(row_prefix);
+      upload(c, A.meta, pk.host);
+      A.djobs = view<JobDesc>(A.meta, oj);
+      dtiles = view<TileDesc>(A.meta, ot);
+      drow_prefix = view<std::uint64_t>(A.meta, orp);
+    }
     pix = DBuf<std::uint64_t>(c.arena, total_px);
     row_cells = DBuf<std::uint64_t>(c.arena, total_rows);
     row_max = DBuf<std::uint32_t>(c.arena, total_rows);
     bstart = DBuf<std::uint64_t>(c.arena, total_bands);
     bcorr = DBuf<std::uint64_t>(c.arena, total_bands);
-    tile_cells = DBuf<std::uint64_t>(c.arena, std::size_t(nt));
-    tile_max = DBuf<std::uint32_t>(c.arena, std::size_t(nt));
-    launch_tile_row_scan(djobs.get(), dtiles.get(), drow_prefix.get(), nt, total_rows, pix.get(), row_cells.get(),
+    tile_counts = DBuf<std::uint64_t>(c.arena, 2 * std::size_t(nt));
+    launch_tile_row_scan(A.djobs.get(), dtiles.get(), drow_prefix.get(), nt, total_rows, pix.get(), row_cells.get(),
                          row_max.get(), bstart.get(), c.stream);
     check_launch();
-    launch_tile_cells(dtiles.get(), nt, row_cells.get(), row_max.get(), bstart.get(), bcorr.get(), tile_cells.get(),
-                      tile_max.get(), c.stream);
+    launch_tile_cells(dtiles.get(), nt, row_cells.get(), row_max.get(), bstart.get(), bcorr.get(), tile_counts.get(),
+                      reinterpret_cast<std::uint32_t*>(tile_counts.get() + nt), c.stream);
     check_launch();
-    tcells.assign(std::size_t(nt), 0);
-    tmax.assign(std::size_t(nt), 0);
-    hip_check(hipMemcpyAsync(tcells.data(), tile_cells.get(), sizeof(std::uint64_t) * nt, hipMemcpyDeviceToHost,
-                             c.stream),
-              "download");
-    hip_check(hipMemcpyAsync(tmax.data(), tile_max.get(), sizeof(std::uint32_t) * nt, hipMemcpyDeviceToHost, c.stream),
+    // Cells and largest label count of every tile, in one download.
+    std::vector<std::uint64_t> counts(2 * std::size_t(nt), 0);
+    hip_check(hipMemcpyAsync(counts.data(), tile_counts.get(), sizeof(std::uint64_t) * 2 * nt,
+                             hipMemcpyDeviceToHost, c.stream),
               "download");
     sync(c);
+    tcells.assign(counts.begin(), counts.begin() + nt);
+    tmax.resize(std::size_t(nt));
+    std::memcpy(tmax.data(), counts.data() + nt, sizeof(std::uint32_t) * nt);
     // Halve the tile size of a job while its largest tile exceeds the memory
     // budget (stereo_matcher.cpp:643-648).
     bool changed = false;
@@ -589,7 +624,7 @@ Aggregation layout_jobs(Context& c, std::vector<Job>& jobs, const r3dg_options& 
     fail(R3DG_ERROR_INVALID_ARGUMENT, "stereo: a tile of " + std::to_string(A.max_tile_cells) +
                                           " cells exceeds the 2^31 cells a tile volume may hold on the GPU; "
                                           "use a smaller tile_size or memory_budget");
-  upload(c, dtiles, td);
+  upload(c, A.dtiles, td);
   layout_timer.reset();
   return A;
 }
@@ -614,9 +649,7 @@ Chunk aggregate_tiles(Context& c, Aggregation& A, int t0, int t1, bool sum_slice
     ch.px += std::uint64_t(T.w) * std::uint64_t(T.h);
     max_num = std::max(max_num, T.max_num);
   }
-  upload(c, ch.dtiles, td);
-  upload(c, ch.drow_prefix, row_prefix);
-  // Group prefixes of the 8 paths (kScanPerWave scanlines per wave), one upload.
+  // Group prefixes of the 8 paths (kScanPerWave scanlines per wave).
   std::vector<std::uint64_t> gp(8 * (std::size_t(nt) + 1), 0);
   std::uint64_t max_groups = 0, sum_groups = 0;
   for (int p = 0; p < 8; ++p) {
@@ -627,8 +660,19 @@ Chunk aggregate_tiles(Context& c, Aggregation& A, int t0, int t1, bool sum_slice
     max_groups = std::max(max_groups, g[nt]);
     sum_groups += g[nt];
   }
-  DBuf<std::uint64_t> dgp;
-  upload(c, dgp, gp);
+  // Tiles, row prefix, group prefixes and the 8 work-queue counters (zero)
+  // in one upload.
+  DView<std::uint64_t> dgp;
+  DView<std::uint32_t> queues;
+  {
+    UploadPack pk;
+    const std::size_t ot = pk.add(td), orp = pk.add(row_prefix), og = pk.add(gp), oq = pk.zeros(8 * sizeof(std::uint32_t));
+    upload(c, ch.meta, pk.host);
+    ch.dtiles = view<TileDesc>(ch.meta, ot);
+    ch.drow_prefix = view<std::uint64_t>(ch.meta, orp);
+    dgp = view<std::uint64_t>(ch.meta, og);
+    queues = view<std::uint32_t>(ch.meta, oq);
+  }
   ch.S = DBuf<std::uint16_t>(c.arena, ch.cells + kSPad);
   // Scratch rows of the persistent waves (pixels wider than the LDS rows) and
   // the work-queue counters of the 8 launches.
@@ -651,8 +695,6 @@ Chunk aggregate_tiles(Context& c, Aggregation& A, int t0, int t1, bool sum_slice
   std::vector<DBuf<std::uint16_t>>& slices = ch.slices;
   for (int w = 1; w < ways; ++w) slices.emplace_back(c.arena, ch.cells + kSPad);
   DBuf<std::uint16_t> scratch(c.arena, std::size_t(ways) * path_scratch_elems(slots, max_num));
-  DBuf<std::uint32_t> queues(c.arena, 8);
-  hip_check(hipMemsetAsync(queues.get(), 0, 8 * sizeof(std::uint32_t), c.stream), "hipMemsetAsync");
 
   // Matching costs of every cell (timed with the other cost preparation).
   DBuf<std::uint8_t> C(c.arena, ch.cells + kSPad);
@@ -754,6 +796,18 @@ std::vector<TilesOut> match_tiles(Context& c, std::vector<Job>& jobs, const r3dg
     check_launch();
   }
   std::vector<TilesOut> out(static_cast<std::size_t>(nj));
+  // Tile grids of every job (extents, cores) in one upload.
+  std::vector<std::int32_t> grid_host;
+  std::vector<std::size_t> grid_cols(static_cast<std::size_t>(nj)), grid_rows(static_cast<std::size_t>(nj));
+  for (int j = 0; j < nj; ++j) {
+    const Grid& g = A.grids[std::size_t(j)];
+    grid_cols[std::size_t(j)] = grid_host.size();
+    for (const auto& e : g.cols) grid_host.insert(grid_host.end(), e.begin(), e.end());
+    grid_rows[std::size_t(j)] = grid_host.size();
+    for (const auto& e : g.rows) grid_host.insert(grid_host.end(), e.begin(), e.end());
+  }
+  DBuf<std::int32_t> dgrids;
+  upload(c, dgrids, grid_host);
   for (int j = 0; j < nj; ++j) {
     const Job& J = jobs[std::size_t(j)];
     const Grid& g = A.grids[std::size_t(j)];
@@ -761,15 +815,9 @@ std::vector<TilesOut> match_tiles(Context& c, std::vector<Job>& jobs, const r3dg
     r.disp = alloc_img<float>(c, J.base->w, J.base->h);
     if (J.confidence) r.conf = alloc_img<float>(c, J.base->w, J.base->h);
     if (J.labels) r.lab = alloc_img<std::int16_t>(c, J.base->w, J.base->h);
-    std::vector<std::int32_t> cols, rows;
-    for (const auto& e : g.cols) cols.insert(cols.end(), e.begin(), e.end());
-    for (const auto& e : g.rows) rows.insert(rows.end(), e.begin(), e.end());
-    DBuf<std::int32_t> dcols, drows;
-    upload(c, dcols, cols);
-    upload(c, drows, rows);
     GridDesc gd{};
-    gd.cols = dcols.get();
-    gd.rows = drows.get();
+    gd.cols = dgrids.get() + grid_cols[std::size_t(j)];
+    gd.rows = dgrids.get() + grid_rows[std::size_t(j)];
     gd.nx = static_cast<int>(g.cols.size());
     gd.ny = static_cast<int>(g.rows.size());
     gd.width = J.base->w;
@@ -787,9 +835,9 @@ std::vector<TilesOut> match_tiles(Context& c, std::vector<Job>& jobs, const r3dg
       r.tile_pixels += std::uint64_t(A.td[std::size_t(t)].w) * std::uint64_t(A.td[std::size_t(t)].h);
     }
     r.tiles = A.tile_base[std::size_t(j) + 1] - A.tile_base[std::size_t(j)];
-    // dcols / drows return to the arena here; later uses of the same memory
-    // are ordered after the blend on the context's stream.
   }
+  // dgrids returns to the arena here; later uses of the same memory are
+  // ordered after the blends on the context's stream.
   return out;
 }
 
