@@ -746,6 +746,71 @@ struct RangePyr {
   int levels;
 };
 
+// Levels 1 .. n (n <= 6) of the min / max pyramid in one launch.  The
+// level-k pixel (x, y) is the min / max over the level-0 pixels [x 2^k,
+// (x + 1) 2^k) x [y 2^k, (y + 1) 2^k) that lie in the grid (what the 2 x 2
+// reduction of k_rng_pyr, repeating the last column and row, gives level by
+// level), so for k <= 6 every pixel lies in one 64 x 64 region of level 0:
+// a block reduces one region, level 1 from global memory and the others in
+// LDS, where pixels outside the grid hold +inf / -inf and drop out of the
+// minimum / maximum.
+constexpr int kPyrRegion = 64;
+constexpr int kPyrLevels = 6;
+struct RangePyrOut {
+  float* mn[kPyrLevels + 1];
+  float* mx[kPyrLevels + 1];
+  int w[kPyrLevels + 1];
+  int h[kPyrLevels + 1];
+  int n;  // levels 1 .. n are written
+};
+__global__ __launch_bounds__(256) void k_rng_pyr_all(const float* __restrict__ cmin, const float* __restrict__ cmax,
+                                                    RangePyrOut P) {
+  constexpr int kS = kPyrRegion / 2;  // level-1 pixels per side of a region
+  __shared__ float smn[kS][kS + 1], smx[kS][kS + 1];
+  const int tx = static_cast<int>(threadIdx.x) & 15, ty = static_cast<int>(threadIdx.x) >> 4;
+  const int rx = static_cast<int>(blockIdx.x) * kPyrRegion, ry = static_cast<int>(blockIdx.y) * kPyrRegion;
+  const int w0 = P.w[0], h0 = P.h[0];
+  auto A = [&](const float* a, int xx, int yy) { return a[std::int64_t(yy) * w0 + xx]; };
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int lx = 2 * tx + (q & 1), ly = 2 * ty + (q >> 1);
+    const int X = (rx >> 1) + lx, Y = (ry >> 1) + ly;
+    float mn = __builtin_huge_valf(), mx = -__builtin_huge_valf();
+    if (X < P.w[1] && Y < P.h[1]) {
+      const int x0 = 2 * X, y0 = 2 * Y, x1 = min(2 * X + 1, w0 - 1), y1 = min(2 * Y + 1, h0 - 1);
+      mn = smin(smin(A(cmin, x0, y0), A(cmin, x1, y0)), smin(A(cmin, x0, y1), A(cmin, x1, y1)));
+      mx = smax(smax(A(cmax, x0, y0), A(cmax, x1, y0)), smax(A(cmax, x0, y1), A(cmax, x1, y1)));
+      P.mn[1][std::int64_t(Y) * P.w[1] + X] = mn;
+      P.mx[1][std::int64_t(Y) * P.w[1] + X] = mx;
+    }
+    smn[ly][lx] = mn;
+    smx[ly][lx] = mx;
+  }
+  for (int k = 2; k <= P.n; ++k) {
+    __syncthreads();
+    const int side = kPyrRegion >> k;
+    const int t = static_cast<int>(threadIdx.x);
+    const int lx = t % side, ly = t / side;
+    float mn = __builtin_huge_valf(), mx = -__builtin_huge_valf();
+    const bool mine = t < side * side;
+    if (mine) {
+      mn = smin(smin(smn[2 * ly][2 * lx], smn[2 * ly][2 * lx + 1]), smin(smn[2 * ly + 1][2 * lx], smn[2 * ly + 1][2 * lx + 1]));
+      mx = smax(smax(smx[2 * ly][2 * lx], smx[2 * ly][2 * lx + 1]), smax(smx[2 * ly + 1][2 * lx], smx[2 * ly + 1][2 * lx + 1]));
+    }
+    __syncthreads();
+    if (mine) {
+      const int X = (rx >> k) + lx, Y = (ry >> k) + ly;
+      const bool in = X < P.w[k] && Y < P.h[k];
+      smn[ly][lx] = in ? mn : __builtin_huge_valf();
+      smx[ly][lx] = in ? mx : -__builtin_huge_valf();
+      if (in) {
+        P.mn[k][std::int64_t(Y) * P.w[k] + X] = mn;
+        P.mx[k][std::int64_t(Y) * P.w[k] + X] = mx;
+      }
+    }
+  }
+}
+
 // Pixels without a valid coarse disparity take the min / max of the first
 // pyramid level with a valid value in the 3x3 block neighbourhood.
 __global__ void k_rng_fill(float* __restrict__ cmin, float* __restrict__ cmax, int wc, int hc, RangePyr pyr) {
@@ -974,6 +1039,10 @@ void launch_ranges_from_coarse(const float* coarse, int wc, int hc, int width, i
   pyr.mn[0] = cmin;
   pyr.mx[0] = cmax;
   float* next = scratch + 4 * a;
+  // Levels 1 .. 6 in one launch, any further ones (range_fill above 64) one
+  // launch each.
+  RangePyrOut po{};
+  po.n = std::min(pyr.levels - 1, kPyrLevels);
   for (int k = 0; k < pyr.levels; ++k) {
     pyr.w[k] = ws[k];
     pyr.h[k] = hs[k];
@@ -981,10 +1050,23 @@ void launch_ranges_from_coarse(const float* coarse, int wc, int hc, int width, i
     float* bmin = next;
     float* bmax = next + std::size_t(ws[k]) * hs[k];
     next = bmax + std::size_t(ws[k]) * hs[k];
-    k_rng_pyr<<<dim3((ws[k] + 255) / 256, hs[k]), 256, 0, s>>>(pyr.mn[k - 1], pyr.mx[k - 1], ws[k - 1], hs[k - 1],
-                                                                ws[k], hs[k], bmin, bmax);
+    if (k > kPyrLevels)
+      k_rng_pyr<<<dim3((ws[k] + 255) / 256, hs[k]), 256, 0, s>>>(pyr.mn[k - 1], pyr.mx[k - 1], ws[k - 1],
+                                                                  hs[k - 1], ws[k], hs[k], bmin, bmax);
     pyr.mn[k] = bmin;
     pyr.mx[k] = bmax;
+    if (k <= kPyrLevels) {
+      po.mn[k] = bmin;
+      po.mx[k] = bmax;
+    }
+    if (k == po.n) {  // the last level of the one-launch part
+      for (int j = 0; j <= po.n; ++j) {
+        po.w[j] = ws[j];
+        po.h[j] = hs[j];
+      }
+      k_rng_pyr_all<<<dim3((wc + kPyrRegion - 1) / kPyrRegion, (hc + kPyrRegion - 1) / kPyrRegion), 256, 0, s>>>(
+          cmin, cmax, po);
+    }
   }
   if (pyr.levels > 1) k_rng_fill<<<gc, 256, 0, s>>>(cmin, cmax, wc, hc, pyr);
   k_rng_final<<<dim3((width + 255) / 256, height), 256, 0, s>>>(cmin, cmax, wc, hc, width, height, fine_valid, fvs,

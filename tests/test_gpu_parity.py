@@ -179,6 +179,27 @@ def test_ranges_from_coarse(gpu, mirrored, sign):
     same_bits(a[1], b[1])
 
 
+@pytest.mark.parametrize("fill", [8, 64, 256])
+def test_ranges_from_coarse_fill_levels(gpu, fill):
+    """A coarse grid of several 64 x 64 regions with a hole that only high
+    min / max pyramid levels reach across: levels up to 6 are built in one
+    launch, those past 6 (range_fill 256) one launch each."""
+    rng = np.random.default_rng(11)
+    wc, hc = 600, 400
+    yy, xx = np.mgrid[0:hc, 0:wc]
+    coarse = (25 + 12 * np.sin(xx / 23.0) + 8 * np.cos(yy / 17.0) + rng.uniform(0, 1, (hc, wc))).astype(np.float32)
+    coarse[rng.random(coarse.shape) < 0.2] = np.inf
+    coarse[5:395, 10:590] = np.inf
+    w, h = 2 * wc, 2 * hc - 1
+    fv = np.full((h, w), 255, np.uint8)
+    pv = np.full((h, w), 255, np.uint8)
+    o = StereoOptions(range_radius=3, range_margin=4, range_fill=fill)
+    a = gpu.ranges_from_coarse(coarse, w, h, fv, pv, w, -1, 0, 128, o, False)
+    b = O.ranges_from_coarse(coarse, w, h, fv, pv, w, -1, 0, 128, o, False)
+    same_bits(a[0], b[0])
+    same_bits(a[1], b[1])
+
+
 @pytest.mark.parametrize("seed", [0, 1, 2])
 def test_speckles_and_median(gpu, seed):
     rng = np.random.default_rng(seed)
