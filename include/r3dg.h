@@ -102,7 +102,7 @@ typedef struct r3dg_options {
 /* Stages timed with HIP events when profiling is on (r3dg_set_profiling). */
 enum {
   R3DG_STAGE_PYRAMID = 0,   /* pyramid_down of images and masks */
-  R3DG_STAGE_PREPARE = 1,   /* mirrors, Census, valid intervals */
+  R3DG_STAGE_PREPARE = 1,   /* Census, valid intervals, matching costs */
   R3DG_STAGE_RANGES = 2,    /* ranges_from_coarse */
   R3DG_STAGE_LAYOUT = 3,    /* tile label layout (including its read-back) */
   R3DG_STAGE_AGGREGATE = 4, /* the 8 SGM path kernels */
@@ -288,8 +288,8 @@ int32_t r3dg_poll(r3dg_ctx* ctx, uint64_t ticket);
 /* Waits up to timeout_seconds (< 0: no limit); returns as r3dg_poll. */
 int32_t r3dg_wait(r3dg_ctx* ctx, uint64_t ticket, double timeout_seconds);
 /* Keeps an image (uint16) and its mask (uint8, may be empty) on the device
- * under a non-zero id, with its pyramid, Census and mirrored copies built on
- * first use and reused by every pair that names the id.  Replaces an existing
+ * under a non-zero id, with its pyramids and Census built on first use and
+ * reused by every pair that names the id.  Replaces an existing
  * view of that id. */
 int32_t r3dg_view_put(r3dg_ctx* ctx, uint64_t view, r3dg_image image, r3dg_image valid);
 /* Drops a view; pairs in flight that use it keep it until they finish. */

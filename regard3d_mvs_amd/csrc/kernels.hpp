@@ -22,9 +22,13 @@ void launch_pyramid_u8(const std::uint8_t* in, std::int64_t is, int w, int h, st
 // uint8 image to uint16 = u8 * 257 (io/image_io.hpp:25-26).
 void launch_widen_u8(const std::uint8_t* in, std::int64_t is, int w, int h, std::uint16_t* out, std::int64_t os,
                      hipStream_t s);
-void launch_mirror_u8(const std::uint8_t* in, std::int64_t is, int w, int h, std::uint8_t* out, std::int64_t os,
-                      hipStream_t s);
 // rows[y] = {first, last} valid column (mask >= 128); mask == nullptr: {0, w - 1}.
+// The row intervals of a level's left and right masks (nullptr: all valid),
+// with those of the mirrored pair (mbase / mpartner, nullptr for none) and
+// the packed rows [h][4] of both directions (rows4b with the mirrored pair).
+void launch_level_rows(const std::uint8_t* ml, std::int64_t msl, int wl, const std::uint8_t* mr, std::int64_t msr,
+                       int wr, int h, std::int32_t* lrows, std::int32_t* rrows, std::int32_t* mbase,
+                       std::int32_t* mpartner, std::int32_t* rows4a, std::int32_t* rows4b, hipStream_t s);
 void launch_valid_intervals(const std::uint8_t* mask, std::int64_t ms, int w, int h, std::int32_t* rows,
                             hipStream_t s);
 // pyramid_down<float> (pyramid.cpp:34-85): float sums, * (1 / 256).

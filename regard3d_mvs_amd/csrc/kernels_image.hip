@@ -135,13 +135,6 @@ __global__ void k_pyr_vf(const float* __restrict__ tmp, int h, int dw, int dh, f
   out[y * os + x] = s * (1.0F / 256.0F);
 }
 
-template <typename T>
-__global__ void k_mirror(const T* __restrict__ in, std::int64_t is, int w, int h, T* __restrict__ out,
-                         std::int64_t os) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  if (x >= w) return;
-  out[y * os + x] = in[y * is + (w - 1 - x)];
-}
 
 // valid_intervals (stereo_matcher.cpp:43-67): first and last column with
 // mask >= 128; {0, -1} for a row without one.
@@ -176,6 +169,74 @@ __global__ __launch_bounds__(256) void k_valid_intervals(const std::uint8_t* __r
     const bool any = sl[0] >= 0;
     rows[2 * y] = any ? sf[0] : 0;
     rows[2 * y + 1] = any ? sl[0] : -1;
+  }
+}
+
+// The row intervals of one level in one launch, one block per row: those of
+// the left and right masks (as k_valid_intervals), with the mirrored pair's
+// (the mirror of a row's valid columns [f, l] is [w - 1 - l, w - 1 - f]; a
+// row without one stays {0, -1}) and the packed (base, partner) rows of both
+// matching directions.  mbase == nullptr: no mirrored pair.
+__global__ __launch_bounds__(256) void k_level_rows(const std::uint8_t* __restrict__ ml, std::int64_t msl, int wl,
+                                                   const std::uint8_t* __restrict__ mr, std::int64_t msr, int wr,
+                                                   std::int32_t* __restrict__ lrows, std::int32_t* __restrict__ rrows,
+                                                   std::int32_t* __restrict__ mbase, std::int32_t* __restrict__ mpartner,
+                                                   std::int32_t* __restrict__ rows4a, std::int32_t* __restrict__ rows4b) {
+  const int y = blockIdx.x;
+  auto scan = [&](const std::uint8_t* mask, std::int64_t ms, int w, int& first, int& last) {
+    first = w;
+    last = -1;
+    if (mask == nullptr) {
+      first = 0;
+      last = w - 1;
+      return;
+    }
+    const std::uint8_t* m = mask + y * ms;
+    for (int x = threadIdx.x; x < w; x += 256) {
+      if (m[x] >= 128) {
+        first = min(first, x);
+        last = max(last, x);
+      }
+    }
+  };
+  int lf, ll, rf, rl;
+  scan(ml, msl, wl, lf, ll);
+  scan(mr, msr, wr, rf, rl);
+  __shared__ int s[4][256];
+  s[0][threadIdx.x] = lf;
+  s[1][threadIdx.x] = ll;
+  s[2][threadIdx.x] = rf;
+  s[3][threadIdx.x] = rl;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) {
+      s[0][threadIdx.x] = min(s[0][threadIdx.x], s[0][threadIdx.x + k]);
+      s[1][threadIdx.x] = max(s[1][threadIdx.x], s[1][threadIdx.x + k]);
+      s[2][threadIdx.x] = min(s[2][threadIdx.x], s[2][threadIdx.x + k]);
+      s[3][threadIdx.x] = max(s[3][threadIdx.x], s[3][threadIdx.x + k]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const bool lany = s[1][0] >= 0, rany = s[3][0] >= 0;
+    lf = lany ? s[0][0] : 0;
+    ll = lany ? s[1][0] : -1;
+    rf = rany ? s[2][0] : 0;
+    rl = rany ? s[3][0] : -1;
+    lrows[2 * y] = lf;
+    lrows[2 * y + 1] = ll;
+    rrows[2 * y] = rf;
+    rrows[2 * y + 1] = rl;
+    reinterpret_cast<int4*>(rows4a)[y] = make_int4(lf, ll, rf, rl);
+    if (mbase != nullptr) {
+      const int bf = rany ? wr - 1 - rl : 0, bl = rany ? wr - 1 - rf : -1;
+      const int pf = lany ? wl - 1 - ll : 0, pl = lany ? wl - 1 - lf : -1;
+      mbase[2 * y] = bf;
+      mbase[2 * y + 1] = bl;
+      mpartner[2 * y] = pf;
+      mpartner[2 * y + 1] = pl;
+      reinterpret_cast<int4*>(rows4b)[y] = make_int4(bf, bl, pf, pl);
+    }
   }
 }
 
@@ -963,14 +1024,16 @@ void launch_widen_u8(const std::uint8_t* in, std::int64_t is, int w, int h, std:
   k_widen_u8<<<dim3((w + 8 * 256 - 1) / (8 * 256), h), 256, 0, s>>>(in, is, w, out, os);
 }
 
-void launch_mirror_u8(const std::uint8_t* in, std::int64_t is, int w, int h, std::uint8_t* out, std::int64_t os,
-                      hipStream_t s) {
-  k_mirror<std::uint8_t><<<dim3((w + 255) / 256, h), 256, 0, s>>>(in, is, w, h, out, os);
-}
-
 void launch_valid_intervals(const std::uint8_t* mask, std::int64_t ms, int w, int h, std::int32_t* rows,
                             hipStream_t s) {
   k_valid_intervals<<<h, 256, 0, s>>>(mask, ms, w, rows);
+}
+
+void launch_level_rows(const std::uint8_t* ml, std::int64_t msl, int wl, const std::uint8_t* mr, std::int64_t msr,
+                       int wr, int h, std::int32_t* lrows, std::int32_t* rrows, std::int32_t* mbase,
+                       std::int32_t* mpartner, std::int32_t* rows4a, std::int32_t* rows4b, hipStream_t s) {
+  if (h <= 0) return;
+  k_level_rows<<<h, 256, 0, s>>>(ml, msl, wl, mr, msr, wr, lrows, rrows, mbase, mpartner, rows4a, rows4b);
 }
 
 void launch_fill_f32(float* p, std::size_t n, float v, hipStream_t s) {

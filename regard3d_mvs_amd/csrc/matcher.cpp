@@ -270,14 +270,6 @@ DImg<std::uint8_t> pyramid_u8(Context& c, const DImg<std::uint8_t>& img, bool th
   return out;
 }
 
-DImg<std::uint8_t> mirror_u8(Context& c, const DImg<std::uint8_t>& img) {
-  if (img.empty()) return DImg<std::uint8_t>{};
-  DImg<std::uint8_t> out = alloc_img<std::uint8_t>(c, img.w, img.h);
-  launch_mirror_u8(img.p, img.s, img.w, img.h, out.p, out.s, c.stream);
-  check_launch();
-  return out;
-}
-
 DBuf<std::int32_t> intervals(Context& c, const DImg<std::uint8_t>& mask, int w, int h) {
   DBuf<std::int32_t> rows(c.arena, 2 * std::size_t(h));
   launch_valid_intervals(mask.empty() ? nullptr : mask.p, mask.s, w, h, rows.get(), c.stream);
@@ -337,7 +329,6 @@ struct View {
   std::deque<DImg<std::uint16_t>> pyr;  // level l at pyr[l - 1]
   std::deque<DImg<std::uint8_t>> mpyr;
   std::map<int, DImg<std::uint64_t>> cen;
-  std::map<int, DImg<std::uint8_t>> mir_mask;
 
   const DImg<std::uint16_t>& image(Context& c, int l) {
     while (static_cast<int>(pyr.size()) < l) pyr.push_back(pyramid_u16(c, pyr.empty() ? img : pyr.back()));
@@ -353,18 +344,12 @@ struct View {
     if (it == cen.end()) it = cen.emplace(l, census(c, image(c, l))).first;
     return it->second;
   }
-  const DImg<std::uint8_t>& mirrored_valid(Context& c, int l) {
-    auto it = mir_mask.find(l);
-    if (it == mir_mask.end()) it = mir_mask.emplace(l, mirror_u8(c, valid(c, l))).first;
-    return it->second;
-  }
   // Device bytes held (cache accounting).
   [[nodiscard]] std::uint64_t bytes() const {
     std::uint64_t b = 2 * img.pixels() + mask.pixels();
     for (const auto& x : pyr) b += 2 * x.pixels();
     for (const auto& x : mpyr) b += x.pixels();
     for (const auto& [l, x] : cen) b += 8 * x.pixels();
-    for (const auto& [l, x] : mir_mask) b += x.pixels();
     return b;
   }
 };
@@ -913,8 +898,19 @@ std::unique_ptr<LevelPrep> prepare_level(Context& c, View& Lv, View& Rv, int l, 
   P->full = o.lr_max_diff >= 0 && o.lr_full_match != 0;
 
   const StageTimer prep_timer(c, R3DG_STAGE_PREPARE);
-  P->lrows = intervals(c, *in.mL, wl, h);
-  P->rrows = intervals(c, *in.mR, wr, h);
+  // Row intervals of both images (and of the mirrored pair), one launch.
+  P->lrows = DBuf<std::int32_t>(c.arena, 2 * std::size_t(h));
+  P->rrows = DBuf<std::int32_t>(c.arena, 2 * std::size_t(h));
+  DBuf<std::int32_t> rows4a(c.arena, 4 * std::size_t(h)), rows4b;
+  if (P->full) {
+    P->mrows_base = DBuf<std::int32_t>(c.arena, 2 * std::size_t(h));
+    P->mrows_partner = DBuf<std::int32_t>(c.arena, 2 * std::size_t(h));
+    rows4b = DBuf<std::int32_t>(c.arena, 4 * std::size_t(h));
+  }
+  launch_level_rows(in.mL->empty() ? nullptr : in.mL->p, in.mL->s, wl, in.mR->empty() ? nullptr : in.mR->p, in.mR->s,
+                    wr, h, P->lrows.get(), P->rrows.get(), P->mrows_base.get(), P->mrows_partner.get(), rows4a.get(),
+                    rows4b.get(), c.stream);
+  check_launch();
   if (coarse_left != nullptr) {
     const StageTimer timer(c, R3DG_STAGE_RANGES);
     P->lranges = ranges_from_coarse(c, *coarse_left, wl, h, *in.mL, P->rrows.get(), in.dmin, in.nd, o, -1, false);
@@ -927,7 +923,7 @@ std::unique_ptr<LevelPrep> prepare_level(Context& c, View& Lv, View& Rv, int l, 
   a.cr = view_of(Rv.census_at(c, l));
   a.lrows = P->lrows.get();
   a.rrows = P->rrows.get();
-  a.rows4 = pack_rows(c, P->lrows.get(), P->rrows.get(), h);
+  a.rows4 = std::move(rows4a);
   a.ranges = coarse_left != nullptr ? &P->lranges : nullptr;
   a.dmin = in.dmin;
   a.D = D;
@@ -936,8 +932,6 @@ std::unique_ptr<LevelPrep> prepare_level(Context& c, View& Lv, View& Rv, int l, 
   if (P->full) {
     // Right-to-left match on the mirrored pair (stereo_matcher.cpp:842-856):
     // base = mirror(R), partner = mirror(L), disparity offset wr - wl.
-    P->mrows_base = intervals(c, Rv.mirrored_valid(c, l), wr, h);
-    P->mrows_partner = intervals(c, Lv.mirrored_valid(c, l), wl, h);
     if (coarse_right != nullptr) {
       const StageTimer timer(c, R3DG_STAGE_RANGES);
       P->rranges = ranges_from_coarse(c, *coarse_right, wr, h, *in.mR, P->lrows.get(), in.dmin, in.nd, o, 1, true);
@@ -952,7 +946,7 @@ std::unique_ptr<LevelPrep> prepare_level(Context& c, View& Lv, View& Rv, int l, 
     b.mirrored = true;
     b.lrows = P->mrows_base.get();
     b.rrows = P->mrows_partner.get();
-    b.rows4 = pack_rows(c, P->mrows_base.get(), P->mrows_partner.get(), h);
+    b.rows4 = std::move(rows4b);
     b.ranges = coarse_right != nullptr ? &P->rranges : nullptr;
     b.dmin = in.dmin + wr - wl;
     b.D = D;
