@@ -35,7 +35,6 @@
 namespace r3dg {
 namespace {
 
-constexpr float kInf = __builtin_huge_valf();
 
 int align16(int n) { return (n + 15) / 16 * 16; }
 // stereo_matcher.cpp:889-890
@@ -1255,6 +1254,7 @@ class Server {
     bool evict = false;
     int w = 0, h = 0, mw = 0, mh = 0;
     std::vector<std::uint16_t> img;
+    std::vector<std::uint8_t> img8;  // uint8 image (R3DG_FORMAT_U8X257), widened on the device
     std::vector<std::uint8_t> mask;
   };
 
@@ -1331,22 +1331,153 @@ class Server {
   }
 
  private:
-  void complete(Request* r, int32_t status) {
+  // Finished pairs handed to the copier: the events before (on the stream)
+  // and after (on the copy stream) their downloads, the staged maps and
+  // blocks, and the device maps the downloads read.
+  struct Publish {
+    Request* r = nullptr;
+    hipEvent_t ready = nullptr, done = nullptr;
+    std::vector<Context::StagedOut> outs;
+    std::vector<void*> blocks;
+    DImg<float> disp, conf;
+  };
+
+  // A finished pair's device state and admission share released (loop thread).
+  void release(Request* r) {
     r->run = PairRun{};
     r->lv.reset();
     r->rv.reset();
+    {
+      const std::lock_guard<std::mutex> lock(m_);
+      in_use_ -= r->footprint;
+    }
+    hold_ = false;  // memory was returned: admission may resume
+  }
+  // The pair's status made visible to its waiter, once its maps are in place.
+  void publish(Request* r, int32_t status) {
     {
       const std::lock_guard<std::mutex> lock(m_);
       r->pair->info.seconds =
           std::chrono::duration<double>(std::chrono::steady_clock::now() - r->submitted).count();
       r->pair->status = status;
       r->status = status;
-      in_use_ -= r->footprint;
       ++completed_;
       if (status != R3DG_OK) ++failed_;
     }
-    hold_ = false;  // memory was returned: admission may resume
     done_cv_.notify_all();
+  }
+  void complete(Request* r, int32_t status) {
+    release(r);
+    publish(r, status);
+  }
+
+  // Finished pairs' maps leave on the copy stream while the loop runs its
+  // next steps.  The copier waits for a pair's downloads, copies the maps it
+  // staged in pinned blocks to the caller's buffers, publishes the pair, and
+  // hands the blocks and the device maps back to the loop thread (returned_,
+  // retired_), which owns the pool and the arena.
+  void copier() {
+    for (;;) {
+      Publish p;
+      {
+        std::unique_lock<std::mutex> lock(pm_);
+        pcv_.wait(lock, [&] { return copier_stop_ || !publish_.empty(); });
+        if (publish_.empty()) return;
+        p = std::move(publish_.front());
+        publish_.pop_front();
+      }
+      int32_t status = R3DG_OK;
+      if (p.done != nullptr && hipEventSynchronize(p.done) != hipSuccess) status = R3DG_ERROR_DEVICE;
+      destroy_events(p);
+      if (status == R3DG_OK)
+        for (const Context::StagedOut& o : p.outs)
+          copy_rows(o.dst, o.dst_stride, o.pinned, o.row_bytes, o.row_bytes, o.rows);
+      Request* r = p.r;
+      {
+        const std::lock_guard<std::mutex> lock(pm_);
+        returned_.insert(returned_.end(), p.blocks.begin(), p.blocks.end());
+        p.blocks.clear();
+        retired_.push_back(std::move(p));
+      }
+      retired_n_.fetch_add(1, std::memory_order_release);
+      cv_.notify_all();  // the loop frees the maps
+      publish(r, status);
+    }
+  }
+  static void destroy_events(Publish& p) {
+    if (p.done != nullptr) (void)hipEventDestroy(p.done);
+    if (p.ready != nullptr) (void)hipEventDestroy(p.ready);
+    p.done = p.ready = nullptr;
+  }
+  // Downloads of a finished pair's map on the copy stream: into a pinned
+  // block (copied to the caller's buffer by the copier), or straight into the
+  // caller's device or page-locked buffer.
+  void download(Publish& p, const r3dg_image& v, const DImg<float>& m, const char* name) {
+    if (view_empty(v)) return;
+    if (v.width != m.w || v.height != m.h)
+      fail(R3DG_ERROR_INVALID_ARGUMENT, std::string(name) + ": output size " + std::to_string(v.width) + "x" +
+                                            std::to_string(v.height) + " differs from " + std::to_string(m.w) + "x" +
+                                            std::to_string(m.h));
+    if (v.stride < v.width) fail(R3DG_ERROR_INVALID_ARGUMENT, std::string(name) + ": stride smaller than width");
+    const std::size_t row = std::size_t(m.w) * sizeof(float), rows = std::size_t(m.h), bytes = row * rows;
+    if (v.memory != R3DG_DEVICE && bytes >= (std::size_t{1} << 20) && !host_pinned(v.data)) {
+      if (void* pin = c_.pinned.get(bytes)) {
+        p.blocks.push_back(pin);
+        hip_check(hipMemcpy2DAsync(pin, row, m.p, std::size_t(m.s) * sizeof(float), row, rows,
+                                   hipMemcpyDeviceToHost, cs_),
+                  "download");
+        p.outs.push_back({pin, v.data, row, rows, std::size_t(v.stride) * sizeof(float)});
+        return;
+      }
+    }
+    const hipMemcpyKind kind = v.memory == R3DG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    hip_check(hipMemcpy2DAsync(v.data, std::size_t(v.stride) * sizeof(float), m.p, std::size_t(m.s) * sizeof(float),
+                               row, rows, kind, cs_),
+              "download");
+  }
+  // Host rows to device memory through a pinned block of the context's pool
+  // (one transfer at full speed instead of the runtime's pageable chunks, and
+  // the host rows are free on return); the block returns to the pool at the
+  // next wait on the stream.
+  void staged_upload(void* dst, const void* src, std::size_t row_bytes, std::size_t rows) {
+    const std::size_t bytes = row_bytes * rows;
+    void* pin = c_.pinned.get(bytes);
+    if (pin == nullptr) fail(R3DG_ERROR_OUT_OF_MEMORY, "view: no page-locked staging memory");
+    c_.staged_blocks.push_back(pin);
+    copy_rows(pin, row_bytes, src, row_bytes, row_bytes, rows);
+    hip_check(hipMemcpyAsync(dst, pin, bytes, hipMemcpyHostToDevice, c_.stream), "upload");
+  }
+  // Pinned blocks the copier is done with, back into the context's pool
+  // (the loop thread, holding the context lock).
+  void take_back_blocks() {
+    std::vector<void*> back;
+    std::vector<Publish> done;
+    {
+      const std::lock_guard<std::mutex> lock(pm_);
+      back.swap(returned_);
+      done.swap(retired_);
+      retired_n_.store(0, std::memory_order_relaxed);
+    }
+    for (void* b : back) c_.pinned.put(b);
+    done.clear();  // device maps back to the arena
+  }
+
+  void loop() {
+    (void)hipSetDevice(c_.device);
+    if (hipStreamCreateWithFlags(&cs_, hipStreamNonBlocking) != hipSuccess) cs_ = nullptr;
+    std::thread cp([this] { copier(); });
+    run_steps();
+    {
+      const std::lock_guard<std::mutex> lock(pm_);
+      copier_stop_ = true;
+    }
+    pcv_.notify_all();
+    cp.join();
+    const std::lock_guard<std::mutex> ctx_lock(c_.mutex);
+    take_back_blocks();
+    (void)guarded(&c_, [&] { sync(c_); });  // the last steps' staging blocks back to the pool
+    if (cs_ != nullptr) (void)hipStreamDestroy(cs_);
+    cs_ = nullptr;
   }
 
   std::shared_ptr<View> view_for(std::uint64_t id, const r3dg_image& img, const r3dg_image& mask, const char* name) {
@@ -1362,14 +1493,18 @@ class Server {
     return v;
   }
 
-  void loop() {
+  // Steps until the loop is stopped and has nothing left to run.
+  void run_steps() {
     std::vector<Request*> active;
     for (;;) {
       std::vector<Request*> admit;
       std::deque<ViewOp> ops;
       {
         std::unique_lock<std::mutex> lock(m_);
-        cv_.wait(lock, [&] { return stop_ || !pending_.empty() || !active.empty() || !view_ops_.empty(); });
+        cv_.wait(lock, [&] {
+          return stop_ || !pending_.empty() || !active.empty() || !view_ops_.empty() ||
+                 retired_n_.load(std::memory_order_acquire) > 0;
+        });
         if (stop_ && pending_.empty() && active.empty() && view_ops_.empty()) return;
         ops.swap(view_ops_);
         // Admission by estimated footprint against the budget left beside
@@ -1389,6 +1524,10 @@ class Server {
       }
       const std::lock_guard<std::mutex> ctx_lock(c_.mutex);
       (void)hipSetDevice(c_.device);
+      take_back_blocks();
+      // View puts: their uploads queued on the stream from pinned copies (no
+      // wait: the cache holds a view from the put on, and later kernels are
+      // ordered after its upload).
       for (ViewOp& op : ops) {
         (void)guarded(&c_, [&] {
           ViewCache& vc = view_cache(c_);
@@ -1397,15 +1536,20 @@ class Server {
             return;
           }
           auto v = std::make_shared<View>();
-          v->img = alloc_img<std::uint16_t>(c_, op.w, op.h);
-          hip_check(hipMemcpyAsync(v->img.p, op.img.data(), op.img.size() * 2, hipMemcpyHostToDevice, c_.stream),
-                    "upload");
+          if (!op.img8.empty()) {
+            const DImg<std::uint8_t> u = alloc_img<std::uint8_t>(c_, op.w, op.h);
+            staged_upload(u.p, op.img8.data(), std::size_t(op.w), std::size_t(op.h));
+            v->img = alloc_img<std::uint16_t>(c_, op.w, op.h);
+            launch_widen_u8(u.p, u.s, u.w, u.h, v->img.p, v->img.s, c_.stream);
+            check_launch();
+          } else {
+            v->img = alloc_img<std::uint16_t>(c_, op.w, op.h);
+            staged_upload(v->img.p, op.img.data(), std::size_t(op.w) * 2, std::size_t(op.h));
+          }
           if (!op.mask.empty()) {
             v->mask = alloc_img<std::uint8_t>(c_, op.mw, op.mh);
-            hip_check(hipMemcpyAsync(v->mask.p, op.mask.data(), op.mask.size(), hipMemcpyHostToDevice, c_.stream),
-                      "upload");
+            staged_upload(v->mask.p, op.mask.data(), std::size_t(op.mw), std::size_t(op.mh));
           }
-          sync(c_);
           vc[op.id] = std::move(v);
         });
       }
@@ -1451,15 +1595,28 @@ class Server {
           armed = true;
         }
       }
+      // Finished pairs' maps are downloaded on the copy stream after the
+      // step's kernels, overlapping the next steps; the copier completes them.
+      std::vector<Publish> pubs;
       const int32_t st = guarded(&c_, [&] {
         match_step(c_, runs, o_);
         for (Request* r : active) {
           if (r->run.next >= 0) continue;
+          if (cs_ == nullptr) fail(R3DG_ERROR_DEVICE, "serve: no copy stream");
           r3dg_pair& q = *r->pair;
-          const TilesOut& m = r->run.res.final.left;
-          output_img(c_, q.disparity, m.disp.p, m.disp.w, m.disp.h, m.disp.s, "disparity");
-          if (!view_empty(q.confidence))
-            output_img(c_, q.confidence, m.conf.p, m.conf.w, m.conf.h, m.conf.s, "confidence");
+          TilesOut& m = r->run.res.final.left;
+          pubs.emplace_back();
+          Publish& last = pubs.back();
+          last.r = r;
+          hip_check(hipEventCreateWithFlags(&last.ready, hipEventDisableTiming), "hipEventCreate");
+          hip_check(hipEventRecord(last.ready, c_.stream), "hipEventRecord");
+          hip_check(hipStreamWaitEvent(cs_, last.ready, 0), "hipStreamWaitEvent");
+          download(last, q.disparity, m.disp, "disparity");
+          if (!view_empty(q.confidence)) download(last, q.confidence, m.conf, "confidence");
+          hip_check(hipEventCreateWithFlags(&last.done, hipEventDisableTiming), "hipEventCreate");
+          hip_check(hipEventRecord(last.done, cs_), "hipEventRecord");
+          last.disp = std::move(m.disp);  // alive until the copy stream is done with it
+          last.conf = std::move(m.conf);
           q.info.disparity_min = q.disparity_min;
           q.info.num_disparities = align16(q.num_disparities);
           q.info.levels = r->run.res.levels;
@@ -1467,9 +1624,23 @@ class Server {
           q.info.cells = r->run.res.cells;
           q.info.tile_pixels = r->run.res.tile_pixels;
         }
-        sync(c_);
+        // No wait here: the next iteration's admission (host staging of new
+        // pairs' images) overlaps this step's last kernels, and the next
+        // step's layout waits for them.  Blocks released meanwhile are reused
+        // in stream order.
       });
       if (armed) c_.arena.inject_failure(-1);  // one the step did not reach is dropped
+      if (st != R3DG_OK) {
+        // Nothing to publish: once the copy stream is idle the blocks and
+        // maps go back, and the pairs complete (or run again) with the step's
+        // status below.
+        if (cs_ != nullptr) (void)hipStreamSynchronize(cs_);
+        for (Publish& p : pubs) {
+          for (void* b : p.blocks) c_.pinned.put(b);
+          destroy_events(p);
+        }
+        pubs.clear();
+      }
       steps_.fetch_add(1, std::memory_order_relaxed);
       if (shortage(st) && active.size() > 1) {
         // Device memory ran out with several pairs in flight: start them
@@ -1487,10 +1658,17 @@ class Server {
         if (st != R3DG_OK) {
           complete(r, st);
         } else if (r->run.next < 0) {
-          complete(r, R3DG_OK);
+          release(r);  // published by the copier
         } else {
           still.push_back(r);
         }
+      }
+      if (!pubs.empty()) {
+        {
+          const std::lock_guard<std::mutex> lock(pm_);
+          for (Publish& p : pubs) publish_.push_back(std::move(p));
+        }
+        pcv_.notify_one();
       }
       active.swap(still);
       active_n_.store(active.size(), std::memory_order_relaxed);
@@ -1553,6 +1731,14 @@ class Server {
   std::unordered_map<std::uint64_t, std::unique_ptr<Request>> reqs_;
   std::uint64_t next_ = 1, in_use_ = 0;
   bool stop_ = false;
+  hipStream_t cs_ = nullptr;     // copy stream (created and destroyed by the loop thread)
+  std::mutex pm_;
+  std::condition_variable pcv_;
+  std::deque<Publish> publish_;  // under pm_
+  std::vector<void*> returned_;  // under pm_: blocks the copier is done with
+  std::vector<Publish> retired_; // under pm_: published pairs whose device maps the loop frees
+  std::atomic<int> retired_n_{0};
+  bool copier_stop_ = false;     // under pm_
   std::thread worker_;
 };
 
@@ -1812,6 +1998,12 @@ DBuf<std::uint64_t> offsets_to_device(Context& c, const std::size_t* off, std::s
 }  // namespace r3dg
 
 using namespace r3dg;
+
+// The context's request loop, or nullptr (without waiting for a loop step).
+static std::shared_ptr<Server> running_server(Context& c) {
+  const std::lock_guard<std::mutex> lock(c.server_mutex);
+  return std::static_pointer_cast<Server>(c.server);
+}
 
 extern "C" {
 
@@ -2604,14 +2796,6 @@ int32_t r3dg_server_start(r3dg_ctx* h, const r3dg_options* options, uint64_t hbm
   });
 }
 
-namespace {
-// The context's request loop, or nullptr (without waiting for a loop step).
-std::shared_ptr<Server> running_server(Context& c) {
-  const std::lock_guard<std::mutex> lock(c.server_mutex);
-  return std::static_pointer_cast<Server>(c.server);
-}
-}  // namespace
-
 int32_t r3dg_server_stop(r3dg_ctx* h) {
   if (h == nullptr) return with_ctx(h, [](Context&) {});
   auto* c = reinterpret_cast<Context*>(h);
@@ -2690,17 +2874,14 @@ int32_t r3dg_view_put(r3dg_ctx* h, uint64_t view, r3dg_image image, r3dg_image v
         op.id = view;
         op.w = image.width;
         op.h = image.height;
-        op.img.resize(std::size_t(image.width) * std::size_t(image.height));
-        for (int y = 0; y < image.height; ++y) {
-          std::uint16_t* dst = op.img.data() + std::size_t(y) * std::size_t(image.width);
-          if (image.format == R3DG_FORMAT_U8X257) {
-            // uint8 rows widened to u8 * 257 here; the loop uploads uint16.
-            const auto* src = static_cast<const std::uint8_t*>(image.data) + std::size_t(y) * std::size_t(image.stride);
-            for (int x = 0; x < image.width; ++x) dst[x] = static_cast<std::uint16_t>(src[x] * 257U);
-          } else {
-            std::memcpy(dst, static_cast<const std::uint16_t*>(image.data) + std::size_t(y) * std::size_t(image.stride),
-                        std::size_t(image.width) * 2);
-          }
+        const std::size_t w = std::size_t(image.width), hh = std::size_t(image.height);
+        if (image.format == R3DG_FORMAT_U8X257) {
+          // uint8 rows, uploaded as bytes and widened to u8 * 257 on the device.
+          op.img8.resize(w * hh);
+          copy_rows(op.img8.data(), w, image.data, std::size_t(image.stride), w, hh);
+        } else {
+          op.img.resize(w * hh);
+          copy_rows(op.img.data(), w * 2, image.data, std::size_t(image.stride) * 2, w * 2, hh);
         }
         if (!view_empty(valid)) {
           if (valid.width != image.width || valid.height != image.height || valid.stride < valid.width)

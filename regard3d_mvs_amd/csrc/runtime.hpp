@@ -188,7 +188,7 @@ inline void copy_rows(void* dst, std::size_t dst_stride, const void* src, std::s
       std::memcpy(static_cast<char*>(dst) + r * dst_stride, static_cast<const char*>(src) + r * src_stride, row_bytes);
   };
   const std::size_t total = row_bytes * rows;
-  const std::size_t nt = std::min<std::size_t>(8, std::max<std::size_t>(1, total >> 23));
+  const std::size_t nt = std::min<std::size_t>(8, std::max<std::size_t>(1, total >> 21));  // 2 MiB or more a thread
   if (nt <= 1 || rows < nt) {
     part(0, rows);
     return;

@@ -91,6 +91,40 @@ def test_serve_cached_views_and_threads():
         same_bits(g.confidence, alone[k % 3].confidence)
 
 
+def test_serve_views_put_while_running():
+    """Views put while the loop runs go up through pinned staging between its
+    steps: uint8 views widened on the device, uint16 views as they are, a view
+    with a mask.  Each served pair equals the same images matched alone."""
+    from regard3d_mvs_amd.native import Matcher
+    p = synth.make_pair(1280, 900, 64, 931)   # uint16 views of 2.3 MB
+    left8 = (p.left >> 8).astype(np.uint8)   # uint8 views of 1.2 MB
+    right8 = (p.right >> 8).astype(np.uint8)
+    mask = np.full(p.left.shape, 255, np.uint8)
+    mask[100:300, 200:500] = 0
+    o = StereoOptions()
+    with Matcher(0) as m:
+        alone = [m.match(p.left, p.right, None, None, p.dmin, p.nd, o),
+                 m.match(left8, right8, None, None, p.dmin, p.nd, o),
+                 m.match(left8, right8, mask, None, p.dmin, p.nd, o)]
+        m.serve_start(o)
+        try:
+            m.view_put(1, p.left)
+            m.view_put(2, p.right)
+            m.view_put(3, left8)
+            m.view_put(4, right8)
+            m.view_put(5, left8, mask)
+            tickets = [m.submit(p.left.shape, p.dmin, p.nd, left_view=1, right_view=2),
+                       m.submit(p.left.shape, p.dmin, p.nd, left_view=3, right_view=4),
+                       m.submit(p.left.shape, p.dmin, p.nd, left_view=5, right_view=4)]
+            got = [m.wait(t) for t in tickets]
+        finally:
+            m.serve_stop()
+    for g, a in zip(got, alone):
+        assert g.cells == a.cells
+        same_bits(g.disparity, a.disparity)
+        same_bits(g.confidence, a.confidence)
+
+
 def test_serve_admission_counts_pairs_from_views():
     """Pairs whose images all come from cached views are admitted by their
     estimated footprint too: with an admission budget of one byte the loop
