@@ -247,8 +247,10 @@ int32_t r3dg_match_batch(r3dg_ctx* ctx, r3dg_pair* pairs, int32_t count, const r
  * context budget) next to the pairs in flight; every step of the loop then
  * matches the next level of every admitted pair with shared launches, so new
  * pairs join while others are on the GPU (continuous batching).  A finished
- * pair's outputs are written and its ticket completes; pair->info.seconds is
- * the time from submission to completion.  A pair and the images it points to
+ * pair's outputs are downloaded on the loop's own copy stream while its next
+ * steps run and written to host buffers by the loop's copier thread; then its
+ * ticket completes; pair->info.seconds is the time from submission to
+ * completion.  A pair and the images it points to
  * must stay valid until its ticket completes.  Each pair's outputs equal
  * those of r3dg_match on it alone.  Direct calls on the context interleave
  * with the loop's steps. */
@@ -290,7 +292,8 @@ int32_t r3dg_wait(r3dg_ctx* ctx, uint64_t ticket, double timeout_seconds);
 /* Keeps an image (uint16) and its mask (uint8, may be empty) on the device
  * under a non-zero id, with its pyramids and Census built on first use and
  * reused by every pair that names the id.  Replaces an existing
- * view of that id. */
+ * view of that id.  While a request loop runs, host images are copied by the
+ * call and uploaded by the loop before its next step. */
 int32_t r3dg_view_put(r3dg_ctx* ctx, uint64_t view, r3dg_image image, r3dg_image valid);
 /* Drops a view; pairs in flight that use it keep it until they finish. */
 int32_t r3dg_view_evict(r3dg_ctx* ctx, uint64_t view);
