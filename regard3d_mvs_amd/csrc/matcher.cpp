@@ -187,6 +187,79 @@ void output_img(Context& c, const r3dg_image& v, const T* src, int w, int h, std
             "download");
 }
 
+// A float map to the caller's buffer on stream cs (a copy stream beside the
+// context's stream): into a pinned block of the context's pool, recorded in
+// outs / blocks for the host copy once cs is past it, or straight into the
+// caller's device or page-locked buffer.
+void download_map(Context& c, hipStream_t cs, const r3dg_image& v, const DImg<float>& m, const char* name,
+                  std::vector<Context::StagedOut>& outs, std::vector<void*>& blocks) {
+  if (view_empty(v)) return;
+  if (v.width != m.w || v.height != m.h)
+    fail(R3DG_ERROR_INVALID_ARGUMENT, std::string(name) + ": output size " + std::to_string(v.width) + "x" +
+                                          std::to_string(v.height) + " differs from " + std::to_string(m.w) + "x" +
+                                          std::to_string(m.h));
+  if (v.stride < v.width) fail(R3DG_ERROR_INVALID_ARGUMENT, std::string(name) + ": stride smaller than width");
+  const std::size_t row = std::size_t(m.w) * sizeof(float), rows = std::size_t(m.h), bytes = row * rows;
+  if (v.memory != R3DG_DEVICE && bytes >= (std::size_t{1} << 20) && !host_pinned(v.data)) {
+    if (void* pin = c.pinned.get(bytes)) {
+      blocks.push_back(pin);
+      hip_check(hipMemcpy2DAsync(pin, row, m.p, std::size_t(m.s) * sizeof(float), row, rows, hipMemcpyDeviceToHost,
+                                 cs),
+                "download");
+      outs.push_back({pin, v.data, row, rows, std::size_t(v.stride) * sizeof(float)});
+      return;
+    }
+  }
+  const hipMemcpyKind kind = v.memory == R3DG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  hip_check(hipMemcpy2DAsync(v.data, std::size_t(v.stride) * sizeof(float), m.p, std::size_t(m.s) * sizeof(float),
+                             row, rows, kind, cs),
+            "download");
+}
+
+// Side stream that carries the downloads of a batch (the path scans use the
+// first two).
+constexpr int kCopySide = 2;
+
+// One batch group's maps on their way to the callers' buffers: the events
+// before (context stream) and after (copy stream) the downloads, the pinned
+// blocks to copy from, and the device maps the downloads read.
+struct Outgoing {
+  explicit Outgoing(Context& ctx) : c(&ctx) {}
+  Outgoing(const Outgoing&) = delete;
+  Outgoing& operator=(const Outgoing&) = delete;
+  ~Outgoing() { release(); }
+  // Waits for the downloads, then copies the staged maps out.
+  void finish() {
+    if (done != nullptr) hip_check(hipEventSynchronize(done), "hipEventSynchronize");
+    for (const Context::StagedOut& o : outs) copy_rows(o.dst, o.dst_stride, o.pinned, o.row_bytes, o.row_bytes, o.rows);
+    outs.clear();
+    release();
+  }
+  // Blocks back to the pool, maps to the arena, events destroyed (after a
+  // failure the context's streams are synchronised before either is reused).
+  void release() {
+    for (void* b : blocks) c->pinned.put(b);
+    blocks.clear();
+    maps.clear();
+    if (done != nullptr) (void)hipEventDestroy(done);
+    if (ready != nullptr) (void)hipEventDestroy(ready);
+    done = ready = nullptr;
+  }
+  Context* c;
+  hipEvent_t ready = nullptr, done = nullptr;
+  std::vector<Context::StagedOut> outs;
+  std::vector<void*> blocks;
+  std::vector<DImg<float>> maps;
+};
+inline void swap(Outgoing& a, Outgoing& b) noexcept {
+  std::swap(a.c, b.c);
+  std::swap(a.ready, b.ready);
+  std::swap(a.done, b.done);
+  a.outs.swap(b.outs);
+  a.blocks.swap(b.blocks);
+  a.maps.swap(b.maps);
+}
+
 // Small host-to-device copy on the context's stream, staged through a pinned
 // block (returned at the next sync), so the host vector may be released right
 // away and the host does not wait for the device.
@@ -1409,31 +1482,10 @@ class Server {
     if (p.ready != nullptr) (void)hipEventDestroy(p.ready);
     p.done = p.ready = nullptr;
   }
-  // Downloads of a finished pair's map on the copy stream: into a pinned
-  // block (copied to the caller's buffer by the copier), or straight into the
-  // caller's device or page-locked buffer.
+  // Downloads of a finished pair's map on the copy stream (pinned blocks are
+  // copied to the caller's buffer by the copier).
   void download(Publish& p, const r3dg_image& v, const DImg<float>& m, const char* name) {
-    if (view_empty(v)) return;
-    if (v.width != m.w || v.height != m.h)
-      fail(R3DG_ERROR_INVALID_ARGUMENT, std::string(name) + ": output size " + std::to_string(v.width) + "x" +
-                                            std::to_string(v.height) + " differs from " + std::to_string(m.w) + "x" +
-                                            std::to_string(m.h));
-    if (v.stride < v.width) fail(R3DG_ERROR_INVALID_ARGUMENT, std::string(name) + ": stride smaller than width");
-    const std::size_t row = std::size_t(m.w) * sizeof(float), rows = std::size_t(m.h), bytes = row * rows;
-    if (v.memory != R3DG_DEVICE && bytes >= (std::size_t{1} << 20) && !host_pinned(v.data)) {
-      if (void* pin = c_.pinned.get(bytes)) {
-        p.blocks.push_back(pin);
-        hip_check(hipMemcpy2DAsync(pin, row, m.p, std::size_t(m.s) * sizeof(float), row, rows,
-                                   hipMemcpyDeviceToHost, cs_),
-                  "download");
-        p.outs.push_back({pin, v.data, row, rows, std::size_t(v.stride) * sizeof(float)});
-        return;
-      }
-    }
-    const hipMemcpyKind kind = v.memory == R3DG_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
-    hip_check(hipMemcpy2DAsync(v.data, std::size_t(v.stride) * sizeof(float), m.p, std::size_t(m.s) * sizeof(float),
-                               row, rows, kind, cs_),
-              "download");
+    download_map(c_, cs_, v, m, name, p.outs, p.blocks);
   }
   // Host rows to device memory through a pinned block of the context's pool
   // (one transfer at full speed instead of the runtime's pageable chunks, and
@@ -2728,6 +2780,7 @@ int32_t r3dg_match_batch(r3dg_ctx* h, r3dg_pair* pairs, int32_t count, const r3d
     // coarsest); the volumes are chunked to the rest by match_tiles.
     const std::uint64_t room = c.arena.budget() > c.arena.live() ? (c.arena.budget() - c.arena.live()) / 2 : 0;
     std::size_t g0 = 0;
+    Outgoing prev(c);
     while (g0 < ok.size()) {
       std::size_t g1 = g0;
       std::uint64_t need = 0, px = 0;
@@ -2749,23 +2802,40 @@ int32_t r3dg_match_batch(r3dg_ctx* h, r3dg_pair* pairs, int32_t count, const r3d
         members.push_back(ok[k]);
       }
       if (!group.empty()) match_batch(c, group, o);
-      for (const int i : members) {
-        r3dg_pair& q = pairs[i];
-        Inputs& x = *in[std::size_t(i)];
-        const TilesOut& m = x.run.res.final.left;
-        output_img(c, q.disparity, m.disp.p, m.disp.w, m.disp.h, m.disp.s, "disparity");
-        if (!view_empty(q.confidence)) output_img(c, q.confidence, m.conf.p, m.conf.w, m.conf.h, m.conf.s, "confidence");
-        q.info.disparity_min = q.disparity_min;
-        q.info.num_disparities = align16(q.num_disparities);
-        q.info.levels = x.run.res.levels;
-        q.info.tiles_level0 = x.run.res.tiles0;
-        q.info.cells = x.run.res.cells;
-        q.info.tile_pixels = x.run.res.tile_pixels;
-        in[std::size_t(i)].reset();
+      // The group's maps go down on a copy stream while the next group is
+      // staged and matched; their host copies follow the next group's
+      // launches (or the last group's).
+      Outgoing cur(c);
+      if (!members.empty()) {
+        const hipStream_t cs = c.side_stream(kCopySide);
+        hip_check(hipEventCreateWithFlags(&cur.ready, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventRecord(cur.ready, c.stream), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(cs, cur.ready, 0), "hipStreamWaitEvent");
+        for (const int i : members) {
+          r3dg_pair& q = pairs[i];
+          Inputs& x = *in[std::size_t(i)];
+          TilesOut& m = x.run.res.final.left;
+          download_map(c, cs, q.disparity, m.disp, "disparity", cur.outs, cur.blocks);
+          if (!view_empty(q.confidence)) download_map(c, cs, q.confidence, m.conf, "confidence", cur.outs, cur.blocks);
+          cur.maps.push_back(std::move(m.disp));
+          cur.maps.push_back(std::move(m.conf));
+          q.info.disparity_min = q.disparity_min;
+          q.info.num_disparities = align16(q.num_disparities);
+          q.info.levels = x.run.res.levels;
+          q.info.tiles_level0 = x.run.res.tiles0;
+          q.info.cells = x.run.res.cells;
+          q.info.tile_pixels = x.run.res.tile_pixels;
+          in[std::size_t(i)].reset();
+        }
+        hip_check(hipEventCreateWithFlags(&cur.done, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventRecord(cur.done, cs), "hipEventRecord");
       }
-      sync(c);
+      prev.finish();
+      swap(prev, cur);
       g0 = g1;
     }
+    prev.finish();
+    sync(c);
     c.collect(stage_seconds, R3DG_STAGES);
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     for (const int i : ok) {
