@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cmath>
@@ -26,6 +27,7 @@
 #include <thread>
 #include <type_traits>
 #include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "kernels.hpp"
@@ -1301,6 +1303,7 @@ struct Request {
   // Worker-side state while admitted.
   std::shared_ptr<View> lv, rv;
   PairRun run;
+  void* pre[2] = {nullptr, nullptr};  // left / right host image copied into a pinned block at admission
 };
 
 // Request loop of a context (r3dg_server_start).
@@ -1417,6 +1420,7 @@ class Server {
 
   // A finished pair's device state and admission share released (loop thread).
   void release(Request* r) {
+    drop_prestaged(r);
     r->run = PairRun{};
     r->lv.reset();
     r->rv.reset();
@@ -1532,7 +1536,8 @@ class Server {
     cs_ = nullptr;
   }
 
-  std::shared_ptr<View> view_for(std::uint64_t id, const r3dg_image& img, const r3dg_image& mask, const char* name) {
+  std::shared_ptr<View> view_for(std::uint64_t id, const r3dg_image& img, const r3dg_image& mask, const char* name,
+                                 void* pre) {
     if (id != 0) {
       ViewCache& vc = view_cache(c_);
       auto it = vc.find(id);
@@ -1540,9 +1545,72 @@ class Server {
       return it->second;
     }
     auto v = std::make_shared<View>();
-    v->img = input_u16(c_, img, name);
+    if (pre != nullptr) {
+      // Already in a pinned block (prestage): one transfer, widened on the device for uint8.
+      c_.staged_blocks.push_back(pre);
+      if (img.format == R3DG_FORMAT_U8X257) {
+        const DImg<std::uint8_t> u = alloc_img<std::uint8_t>(c_, img.width, img.height);
+        hip_check(hipMemcpyAsync(u.p, pre, u.pixels(), hipMemcpyHostToDevice, c_.stream), "upload");
+        v->img = alloc_img<std::uint16_t>(c_, img.width, img.height);
+        launch_widen_u8(u.p, u.s, u.w, u.h, v->img.p, v->img.s, c_.stream);
+        check_launch();
+      } else {
+        v->img = alloc_img<std::uint16_t>(c_, img.width, img.height);
+        hip_check(hipMemcpyAsync(v->img.p, pre, v->img.pixels() * 2, hipMemcpyHostToDevice, c_.stream), "upload");
+      }
+    } else {
+      v->img = input_u16(c_, img, name);
+    }
     v->mask = input_img<std::uint8_t>(c_, mask, "valid");
     return v;
+  }
+  // The host images of the pairs about to be admitted, copied into pinned
+  // blocks by several threads at once (one pair after the other, the copies
+  // would leave the device idle between the pairs' uploads).
+  void prestage(const std::vector<Request*>& admit) {
+    struct Copy {
+      void* pin;
+      const void* src;
+      std::size_t row, stride, rows;
+    };
+    std::vector<Copy> jobs;
+    for (Request* r : admit) {
+      const r3dg_pair& q = *r->pair;
+      const r3dg_image* imgs[2] = {r->left_view == 0 ? &q.left : nullptr, r->right_view == 0 ? &q.right : nullptr};
+      for (int k = 0; k < 2; ++k) {
+        const r3dg_image* v = imgs[k];
+        if (v == nullptr || view_empty(*v) || v->memory != R3DG_HOST || v->stride < v->width || v->data == nullptr)
+          continue;
+        if (v->format != R3DG_FORMAT_NATIVE && v->format != R3DG_FORMAT_U8X257) continue;
+        const std::size_t es = v->format == R3DG_FORMAT_U8X257 ? 1 : 2;
+        const std::size_t row = std::size_t(v->width) * es, rows = std::size_t(v->height);
+        if (row * rows < (std::size_t{1} << 20) || host_pinned(v->data)) continue;
+        void* pin = c_.pinned.get(row * rows);
+        if (pin == nullptr) continue;
+        r->pre[k] = pin;
+        jobs.push_back({pin, v->data, row, std::size_t(v->stride) * es, rows});
+      }
+    }
+    if (jobs.empty()) return;
+    std::atomic<std::size_t> next{0};
+    auto work = [&] {
+      for (std::size_t j; (j = next.fetch_add(1)) < jobs.size();) {
+        const Copy& J = jobs[j];
+        for (std::size_t y = 0; y < J.rows; ++y)
+          std::memcpy(static_cast<char*>(J.pin) + y * J.row, static_cast<const char*>(J.src) + y * J.stride, J.row);
+      }
+    };
+    std::vector<std::thread> th;
+    for (std::size_t k = 1; k < std::min<std::size_t>(8, jobs.size()); ++k) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+  }
+  // Pinned blocks of a request's prestaged images that no upload took.
+  void drop_prestaged(Request* r) {
+    for (void*& b : r->pre) {
+      if (b != nullptr) c_.pinned.put(b);
+      b = nullptr;
+    }
   }
 
   // Steps until the loop is stopped and has nothing left to run.
@@ -1606,6 +1674,7 @@ class Server {
         });
       }
       if (!ops.empty()) update_view_bytes();
+      prestage(admit);
       std::vector<Request*> retry;  // admitted while memory was short: back to the queue
       for (Request* r : admit) {
         if (!retry.empty()) {
@@ -1616,8 +1685,9 @@ class Server {
           r3dg_pair& q = *r->pair;
           std::memset(&q.info, 0, sizeof(q.info));
           if (view_empty(q.disparity)) fail(R3DG_ERROR_INVALID_ARGUMENT, "match: empty disparity output");
-          r->lv = view_for(r->left_view, q.left, q.left_valid, "left");
-          r->rv = view_for(r->right_view, q.right, q.right_valid, "right");
+          // A prestaged block passes to its upload (back to the pool at the next wait).
+          r->lv = view_for(r->left_view, q.left, q.left_valid, "left", std::exchange(r->pre[0], nullptr));
+          r->rv = view_for(r->right_view, q.right, q.right_valid, "right", std::exchange(r->pre[1], nullptr));
           r->run = PairRun{};
           r->run.lv = r->lv.get();
           r->run.rv = r->rv.get();
@@ -1751,6 +1821,7 @@ class Server {
     const std::lock_guard<std::mutex> lock(m_);
     for (auto it = rs.rbegin(); it != rs.rend(); ++it) {
       Request* r = *it;
+      drop_prestaged(r);
       r->run = PairRun{};
       r->lv.reset();
       r->rv.reset();
