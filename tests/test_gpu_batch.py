@@ -83,3 +83,29 @@ def test_batch_invalid_pair_is_skipped(gpu):
 
 def test_batch_empty(gpu):
     assert gpu.match_batch([], StereoOptions()) == []
+
+
+def test_batch_outputs_in_every_memory(gpu):
+    """A group's maps leave on a copy stream: staged through pinned blocks for
+    pageable host buffers of 1 MB or more, straight into page-locked host
+    buffers and into device buffers.  Each form equals the plain call."""
+    import torch
+    from regard3d_mvs_amd.native import host_empty
+    o = StereoOptions()
+    pairs = [synth.make_pair(640, 480, 48, 700 + i) for i in range(3)]
+    ref = [gpu.match(p.left, p.right, None, None, p.dmin, p.nd, o) for p in pairs]
+    pinned = [host_empty((480, 640), np.float32) for _ in range(2)]
+    dev = torch.device("cuda:0")
+    reqs = [dict(left=pairs[0].left, right=pairs[0].right, dmin=pairs[0].dmin, nd=pairs[0].nd),
+            dict(left=pairs[1].left, right=pairs[1].right, dmin=pairs[1].dmin, nd=pairs[1].nd,
+                 out=(pinned[0][0], pinned[1][0])),
+            dict(left=torch.from_numpy(pairs[2].left.astype(np.int32)).to(dev).to(torch.uint16),
+                 right=torch.from_numpy(pairs[2].right.astype(np.int32)).to(dev).to(torch.uint16),
+                 dmin=pairs[2].dmin, nd=pairs[2].nd)]
+    got = gpu.match_batch(reqs, o)
+    for g, r in zip(got, ref):
+        d, c = g.disparity, g.confidence
+        if hasattr(d, "cpu"):
+            d, c = d.cpu().numpy(), c.cpu().numpy()
+        same_bits(d, r.disparity)
+        same_bits(c, r.confidence)
