@@ -275,6 +275,8 @@ typedef struct r3dg_server_stats {
   uint64_t in_use;           /* estimated device bytes of the admitted pairs */
   uint64_t view_bytes;       /* device bytes of the cached views */
   double admission_scale;    /* factor on the admission budget, 1 at start */
+  double loop_seconds;       /* loop thread: time in its steps (admission, launches, waits) */
+  double wait_seconds;       /* of that, time waiting for the device (stream synchronisations) */
 } r3dg_server_stats;
 int32_t r3dg_server_stats_get(r3dg_ctx* ctx, r3dg_server_stats* out);
 /* Test hook, no counterpart in the reference: in the next loop step with at

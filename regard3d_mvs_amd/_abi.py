@@ -205,7 +205,8 @@ class CMatchDebug(ctypes.Structure):
 class CServerStats(ctypes.Structure):
     """r3dg_server_stats: counters of the running request loop."""
     _fields_ = [(k, ctypes.c_uint64) for k in ("submitted", "completed", "failed", "steps", "requeued", "pending",
-                                                "active", "in_use", "view_bytes")] + [("admission_scale", ctypes.c_double)]
+                                                "active", "in_use", "view_bytes")] + [
+        (k, ctypes.c_double) for k in ("admission_scale", "loop_seconds", "wait_seconds")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -634,12 +634,24 @@ Aggregation layout_jobs(Context& c, std::vector<Job>& jobs, const r3dg_options& 
     launch_tile_cells(dtiles.get(), nt, row_cells.get(), row_max.get(), bstart.get(), bcorr.get(), tile_counts.get(),
                       reinterpret_cast<std::uint32_t*>(tile_counts.get() + nt), c.stream);
     check_launch();
-    // Cells and largest label count of every tile, in one download.
+    // Cells and largest label count of every tile, in one download (into a
+    // pinned block, so that the host waits in the stream synchronisation).
     std::vector<std::uint64_t> counts(2 * std::size_t(nt), 0);
-    hip_check(hipMemcpyAsync(counts.data(), tile_counts.get(), sizeof(std::uint64_t) * 2 * nt,
-                             hipMemcpyDeviceToHost, c.stream),
-              "download");
-    sync(c);
+    {
+      const std::size_t cbytes = sizeof(std::uint64_t) * counts.size();
+      struct Block {
+        PinnedPool& pool;
+        void* p;
+        ~Block() {
+          if (p != nullptr) pool.put(p);
+        }
+      } pin{c.pinned, c.pinned.get(cbytes)};
+      hip_check(hipMemcpyAsync(pin.p != nullptr ? pin.p : counts.data(), tile_counts.get(), cbytes,
+                               hipMemcpyDeviceToHost, c.stream),
+                "download");
+      sync(c);
+      if (pin.p != nullptr) std::memcpy(counts.data(), pin.p, cbytes);
+    }
     tcells.assign(counts.begin(), counts.begin() + nt);
     tmax.resize(std::size_t(nt));
     std::memcpy(tmax.data(), counts.data() + nt, sizeof(std::uint32_t) * nt);
@@ -1386,6 +1398,8 @@ class Server {
     s.in_use = in_use_;
     s.view_bytes = view_bytes_.load(std::memory_order_relaxed);
     s.admission_scale = scale_.load(std::memory_order_relaxed);
+    s.loop_seconds = loop_s_.load(std::memory_order_relaxed);
+    s.wait_seconds = wait_s_.load(std::memory_order_relaxed);
     return s;
   }
 
@@ -1643,6 +1657,21 @@ class Server {
         }
       }
       const std::lock_guard<std::mutex> ctx_lock(c_.mutex);
+      const auto busy0 = std::chrono::steady_clock::now();
+      const double wait0 = c_.wait_seconds;
+      // Time of this iteration, added when it ends (any of its exits).
+      struct Clock {
+        Server& s;
+        std::chrono::steady_clock::time_point t0;
+        double w0;
+        ~Clock() {
+          s.loop_s_.store(s.loop_s_.load(std::memory_order_relaxed) +
+                              std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(),
+                          std::memory_order_relaxed);
+          s.wait_s_.store(s.wait_s_.load(std::memory_order_relaxed) + (s.c_.wait_seconds - w0),
+                          std::memory_order_relaxed);
+        }
+      } clock{*this, busy0, wait0};
       (void)hipSetDevice(c_.device);
       take_back_blocks();
       // View puts: their uploads queued on the stream from pinned copies (no
@@ -1844,6 +1873,7 @@ class Server {
   bool hold_ = false;                              // loop thread only: no admission until a completion
   std::atomic<std::uint64_t> view_bytes_{0};       // cached views, updated by the loop
   std::atomic<double> scale_{1.0};                 // budget_scale_ for the counters
+  std::atomic<double> loop_s_{0.0}, wait_s_{0.0};   // loop thread: time in steps, of it waiting for the device
   std::atomic<std::uint64_t> steps_{0}, active_n_{0};
   std::atomic<std::int64_t> step_fail_after_{-1};
   std::uint64_t completed_ = 0, failed_ = 0, requeued_ = 0;  // under m_

@@ -8,6 +8,7 @@
 #include <atomic>
 #include <cstdint>
 #include <thread>
+#include <chrono>
 #include <cstring>
 #include <iterator>
 #include <map>
@@ -279,8 +280,11 @@ struct Context {
   std::vector<StagedOut> staged_out;
   std::vector<void*> staged_blocks;
   // Waits for the stream, then completes the staged outputs.
+  double wait_seconds = 0;  // time spent in sync(), for the request loop's counters
   void sync() {
+    const auto w0 = std::chrono::steady_clock::now();
     const hipError_t e = hipStreamSynchronize(stream);
+    wait_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
     if (e == hipSuccess) {
       for (const StagedOut& o : staged_out) copy_rows(o.dst, o.dst_stride, o.pinned, o.row_bytes, o.row_bytes, o.rows);
     }
