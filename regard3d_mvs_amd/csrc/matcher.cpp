@@ -1672,6 +1672,13 @@ class Server {
                           std::memory_order_relaxed);
         }
       } clock{*this, busy0, wait0};
+      // No stage timing inside the loop (nothing would collect the events);
+      // direct calls, under the same lock, keep the context's setting.
+      struct NoStageTiming {
+        Context& c;
+        int saved;
+        ~NoStageTiming() { c.profiling = saved; }
+      } no_timing{c_, std::exchange(c_.profiling, 0)};
       (void)hipSetDevice(c_.device);
       take_back_blocks();
       // View puts: their uploads queued on the stream from pinned copies (no
