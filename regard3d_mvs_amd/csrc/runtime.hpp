@@ -279,8 +279,8 @@ struct Context {
   };
   std::vector<StagedOut> staged_out;
   std::vector<void*> staged_blocks;
-  // Waits for the stream, then completes the staged outputs.
   double wait_seconds = 0;  // time spent in sync(), for the request loop's counters
+  // Waits for the stream, then completes the staged outputs.
   void sync() {
     const auto w0 = std::chrono::steady_clock::now();
     const hipError_t e = hipStreamSynchronize(stream);
