@@ -460,8 +460,9 @@ def main() -> int:
     if args.cpu_rows <= 0:
         args.cpu_runs = 0
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(pairs are independent, see DESIGN.md for how they would shard)", file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported (pairs are independent: "
+              "they would shard across GPUs, one process each, with pairs sharing a view on one GPU)",
+              file=sys.stderr)
         return 2
     if args.workload == "copy":
         import torch
